@@ -1,0 +1,1077 @@
+/* CPU restatement of the reference's ray-projection path -- TEST INFRASTRUCTURE.
+ *
+ * This is the oracle the HIP product is checked against.  It is not product
+ * code and is never linked into vortrace_amd.  Every function names the
+ * reference lines it restates.  Build: oracle/Makefile (-ffp-contract=off, no
+ * -march, so every FP64 multiply and add rounds separately, as in a default
+ * x86-64 build of the reference).
+ *
+ * Pinning (see tests/test_oracle_*.py):
+ *   - tree build: vind permutation, cut dims and divlow/divhigh equal the
+ *     reference's own nanoflann dump bit for bit (oracle/_ref), on uniform,
+ *     clustered, lattice and duplicate-heavy clouds;
+ *   - kNN: ids and r^2 equal oracle/_ref and a brute-force 27-image search;
+ *   - walk: per-ray segment ids and s bits equal oracle/_ref on lattices
+ *     (degenerate cases), random, clustered and periodic meshes; the golden
+ *     image tests/golden/galaxy_interaction-proj.npy at decimal=14;
+ *   - reductions: closed forms (ray length, C(1-exp(-aL))) and linearity.
+ */
+#define _GNU_SOURCE
+#include "vt_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <omp.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* src/ray.cpp:8-12 and src/pointcloud.cpp:10 */
+#define TOL 1E-9
+#define PAR_TOL 1E-12
+#define MAX_IT 100000
+#define PERT_EPS (2 * TOL)
+#define MAX_CYC 1000
+/* src/pointcloud.cpp:204 */
+#define LEAF_MAX 10
+#define NONE ((size_t)-1)
+
+static __thread char g_err[512];
+
+const char *vto_last_error(void) { return g_err; }
+
+static int set_err(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+typedef struct {
+  int32_t feat; /* -1: leaf */
+  size_t left, right;
+  double lo, hi;
+  int32_t c1, c2;
+} onode;
+
+struct vto_cloud {
+  size_t n, nf;
+  double *pts;    /* n x 3, filtered order (src/pointcloud.cpp:86-108) */
+  double *fields; /* n x nf, particle-major */
+  int64_t *orig_ids;
+  double subbox[6], box[3], pad;
+  int periodic, valid_rgba, tree_built;
+  size_t *vind;
+  onode *nodes;
+  size_t nnodes, cap;
+  double root_bbox[3][2];
+  int nwarn;
+  char warn[4][256];
+};
+
+static void add_warning(vto_cloud *c, const char *m) {
+  if (c->nwarn < 4) snprintf(c->warn[c->nwarn++], 256, "%s", m);
+}
+
+static inline double P(const vto_cloud *c, size_t i, int d) { return c->pts[3 * i + d]; }
+
+/* ---------------------------------------------------------------------------
+ * nanoflann 1.3.2 static index build (include/nanoflann.hpp:1190-1202,
+ * 858-1003, 1307-1339).
+ * ------------------------------------------------------------------------- */
+
+static void min_max(const vto_cloud *c, const size_t *ind, size_t count, int e, double *mn,
+                    double *mx) {
+  /* computeMinMax, nanoflann.hpp:843-856 */
+  *mn = P(c, ind[0], e);
+  *mx = P(c, ind[0], e);
+  for (size_t i = 1; i < count; ++i) {
+    double v = P(c, ind[i], e);
+    if (v < *mn) *mn = v;
+    if (v > *mx) *mx = v;
+  }
+}
+
+static void plane_split(const vto_cloud *c, size_t *ind, size_t count, int f, double cutval,
+                        size_t *lim1, size_t *lim2) {
+  /* planeSplit, nanoflann.hpp:967-1003: two Hoare passes, '<' then '<=' */
+  size_t left = 0, right = count - 1;
+  for (;;) {
+    while (left <= right && P(c, ind[left], f) < cutval) ++left;
+    while (right && left <= right && P(c, ind[right], f) >= cutval) --right;
+    if (left > right || !right) break;
+    size_t t = ind[left];
+    ind[left] = ind[right];
+    ind[right] = t;
+    ++left;
+    --right;
+  }
+  *lim1 = left;
+  right = count - 1;
+  for (;;) {
+    while (left <= right && P(c, ind[left], f) <= cutval) ++left;
+    while (right && left <= right && P(c, ind[right], f) > cutval) --right;
+    if (left > right || !right) break;
+    size_t t = ind[left];
+    ind[left] = ind[right];
+    ind[right] = t;
+    ++left;
+    --right;
+  }
+  *lim2 = left;
+}
+
+static void middle_split(const vto_cloud *c, size_t *ind, size_t count, size_t *index,
+                         int *cutfeat, double *cutval, double bbox[3][2]) {
+  /* middleSplit_, nanoflann.hpp:910-959 */
+  const double EPS = 0.00001;
+  double max_span = bbox[0][1] - bbox[0][0];
+  for (int i = 1; i < 3; ++i) {
+    double span = bbox[i][1] - bbox[i][0];
+    if (span > max_span) max_span = span;
+  }
+  double max_spread = -1;
+  *cutfeat = 0;
+  for (int i = 0; i < 3; ++i) {
+    double span = bbox[i][1] - bbox[i][0];
+    if (span > (1 - EPS) * max_span) {
+      double mn, mx;
+      min_max(c, ind, count, i, &mn, &mx);
+      double spread = mx - mn;
+      if (spread > max_spread) {
+        *cutfeat = i;
+        max_spread = spread;
+      }
+    }
+  }
+  double split_val = (bbox[*cutfeat][0] + bbox[*cutfeat][1]) / 2;
+  double mn, mx;
+  min_max(c, ind, count, *cutfeat, &mn, &mx);
+  if (split_val < mn)
+    *cutval = mn;
+  else if (split_val > mx)
+    *cutval = mx;
+  else
+    *cutval = split_val;
+  size_t lim1, lim2;
+  plane_split(c, ind, count, *cutfeat, *cutval, &lim1, &lim2);
+  if (lim1 > count / 2)
+    *index = lim1;
+  else if (lim2 < count / 2)
+    *index = lim2;
+  else
+    *index = count / 2;
+}
+
+static int32_t new_node(vto_cloud *c) {
+  if (c->nnodes == c->cap) {
+    c->cap = c->cap ? 2 * c->cap : 1024;
+    c->nodes = realloc(c->nodes, c->cap * sizeof(onode));
+  }
+  return (int32_t)c->nnodes++;
+}
+
+static int32_t divide_tree(vto_cloud *c, size_t left, size_t right, double bbox[3][2]) {
+  /* divideTree, nanoflann.hpp:858-907; nodes are stored in pre-order */
+  int32_t id = new_node(c);
+  if (right - left <= LEAF_MAX) {
+    c->nodes[id].feat = -1;
+    c->nodes[id].left = left;
+    c->nodes[id].right = right;
+    c->nodes[id].c1 = c->nodes[id].c2 = -1;
+    for (int i = 0; i < 3; ++i) bbox[i][0] = bbox[i][1] = P(c, c->vind[left], i);
+    for (size_t k = left + 1; k < right; ++k)
+      for (int i = 0; i < 3; ++i) {
+        double v = P(c, c->vind[k], i);
+        if (bbox[i][0] > v) bbox[i][0] = v;
+        if (bbox[i][1] < v) bbox[i][1] = v;
+      }
+    return id;
+  }
+  size_t idx;
+  int cutfeat;
+  double cutval;
+  middle_split(c, c->vind + left, right - left, &idx, &cutfeat, &cutval, bbox);
+  c->nodes[id].feat = cutfeat;
+  double lb[3][2], rb[3][2];
+  memcpy(lb, bbox, sizeof lb);
+  lb[cutfeat][1] = cutval;
+  int32_t c1 = divide_tree(c, left, left + idx, lb);
+  memcpy(rb, bbox, sizeof rb);
+  rb[cutfeat][0] = cutval;
+  int32_t c2 = divide_tree(c, left + idx, right, rb);
+  c->nodes[id].c1 = c1;
+  c->nodes[id].c2 = c2;
+  c->nodes[id].lo = lb[cutfeat][1];
+  c->nodes[id].hi = rb[cutfeat][0];
+  for (int i = 0; i < 3; ++i) {
+    bbox[i][0] = (rb[i][0] < lb[i][0]) ? rb[i][0] : lb[i][0]; /* std::min */
+    bbox[i][1] = (lb[i][1] < rb[i][1]) ? rb[i][1] : lb[i][1]; /* std::max */
+  }
+  return id;
+}
+
+static void build_tree(vto_cloud *c) {
+  /* buildIndex + computeBoundingBox, nanoflann.hpp:1190-1202, 1316-1339 */
+  c->vind = malloc(c->n * sizeof(size_t));
+  for (size_t i = 0; i < c->n; i++) c->vind[i] = i;
+  for (int i = 0; i < 3; ++i) c->root_bbox[i][0] = c->root_bbox[i][1] = P(c, 0, i);
+  for (size_t k = 1; k < c->n; ++k)
+    for (int i = 0; i < 3; ++i) {
+      double v = P(c, k, i);
+      if (v < c->root_bbox[i][0]) c->root_bbox[i][0] = v;
+      if (v > c->root_bbox[i][1]) c->root_bbox[i][1] = v;
+    }
+  double bb[3][2];
+  memcpy(bb, c->root_bbox, sizeof bb);
+  divide_tree(c, 0, c->n, bb);
+  /* divideTree tightens the bbox it is handed, and buildIndex hands it
+   * root_bbox itself; the query uses that tightened box.  For the root the
+   * tight box equals the computed one, so nothing changes, but keep the
+   * assignment to state the data flow. */
+  memcpy(c->root_bbox, bb, sizeof bb);
+  c->tree_built = 1;
+}
+
+/* ---------------------------------------------------------------------------
+ * nanoflann kNN search (include/nanoflann.hpp:142-205, 432-440, 1005-1021,
+ * 1221-1258, 1346-1408).
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+  size_t k, count;
+  size_t *ids;
+  double *d;
+} rset;
+
+static void rs_init(rset *r, size_t k, size_t *ids, double *d) {
+  r->k = k;
+  r->count = 0;
+  r->ids = ids;
+  r->d = d;
+  if (k) d[k - 1] = DBL_MAX;
+}
+
+static void rs_add(rset *r, double dist, size_t index) {
+  /* KNNResultSet::addPoint without NANOFLANN_FIRST_MATCH (175-202) */
+  size_t i;
+  for (i = r->count; i > 0; --i) {
+    if (r->d[i - 1] > dist) {
+      if (i < r->k) {
+        r->d[i] = r->d[i - 1];
+        r->ids[i] = r->ids[i - 1];
+      }
+    } else
+      break;
+  }
+  if (i < r->k) {
+    r->d[i] = dist;
+    r->ids[i] = index;
+  }
+  if (r->count < r->k) r->count++;
+}
+
+static inline double eval_metric(const vto_cloud *c, const double *q, size_t idx) {
+  /* L2_Simple_Adaptor::evalMetric (432-440): ((0 + d0^2) + d1^2) + d2^2 */
+  double result = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    double diff = q[i] - P(c, idx, i);
+    result += diff * diff;
+  }
+  return result;
+}
+
+static void search_level(const vto_cloud *c, rset *r, const double *q, int32_t node,
+                         double mindistsq, double dists[3]) {
+  const onode *nd = &c->nodes[node];
+  if (nd->feat < 0) {
+    double worst = r->d[r->k - 1];
+    for (size_t i = nd->left; i < nd->right; ++i) {
+      size_t index = c->vind[i];
+      double dist = eval_metric(c, q, index);
+      if (dist < worst) rs_add(r, dist, index);
+    }
+    return;
+  }
+  int idx = nd->feat;
+  double val = q[idx];
+  double diff1 = val - nd->lo;
+  double diff2 = val - nd->hi;
+  int32_t best, other;
+  double cut_dist;
+  if ((diff1 + diff2) < 0) {
+    best = nd->c1;
+    other = nd->c2;
+    cut_dist = (val - nd->hi) * (val - nd->hi);
+  } else {
+    best = nd->c2;
+    other = nd->c1;
+    cut_dist = (val - nd->lo) * (val - nd->lo);
+  }
+  search_level(c, r, q, best, mindistsq, dists);
+  double dst = dists[idx];
+  mindistsq = mindistsq + cut_dist - dst;
+  dists[idx] = cut_dist;
+  if (mindistsq * (double)1.0f <= r->d[r->k - 1]) search_level(c, r, q, other, mindistsq, dists);
+  dists[idx] = dst;
+}
+
+static size_t knn_search(const vto_cloud *c, const double *q, size_t k, size_t *ids,
+                         double *d) {
+  /* knnSearch + findNeighbors + computeInitialDistances */
+  rset r;
+  for (size_t i = 0; i < k; i++) {
+    ids[i] = NONE;
+    d[i] = DBL_MAX;
+  }
+  rs_init(&r, k, ids, d);
+  if (c->n == 0) return 0;
+  double dists[3] = {0, 0, 0};
+  double distsq = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (q[i] < c->root_bbox[i][0]) {
+      dists[i] = (q[i] - c->root_bbox[i][0]) * (q[i] - c->root_bbox[i][0]);
+      distsq += dists[i];
+    }
+    if (q[i] > c->root_bbox[i][1]) {
+      dists[i] = (q[i] - c->root_bbox[i][1]) * (q[i] - c->root_bbox[i][1]);
+      distsq += dists[i];
+    }
+  }
+  search_level(c, &r, q, 0, distsq, dists);
+  return r.count;
+}
+
+static double min_dist_sq_to_box(const vto_cloud *c, const double *q) {
+  /* PointCloud::minDistSqToBox, src/pointcloud.cpp:283-298 */
+  double d2 = 0.0;
+  for (int dim = 0; dim < 3; dim++) {
+    double lo = c->subbox[2 * dim], hi = c->subbox[2 * dim + 1];
+    if (q[dim] < lo) {
+      double d = lo - q[dim];
+      d2 += d * d;
+    } else if (q[dim] > hi) {
+      double d = q[dim] - hi;
+      d2 += d * d;
+    }
+  }
+  return d2;
+}
+
+typedef struct {
+  size_t id;
+  double d;
+} iddist;
+
+static void query_k(const vto_cloud *c, const double *q, size_t k, size_t *res, double *r2) {
+  /* PointCloud::queryTreeK, src/pointcloud.cpp:308-368 */
+  knn_search(c, q, k, res, r2);
+  if (!c->periodic) return;
+  iddist stackbuf[16];
+  iddist *merged = 2 * k <= 16 ? stackbuf : malloc(2 * k * sizeof(iddist));
+  size_t *img_ids = malloc(k * sizeof(size_t));
+  double *img_r2 = malloc(k * sizeof(double));
+  size_t m = k;
+  for (size_t i = 0; i < k; i++) {
+    merged[i].id = res[i];
+    merged[i].d = r2[i];
+  }
+  for (int ix = -1; ix <= 1; ix++)
+    for (int iy = -1; iy <= 1; iy++)
+      for (int iz = -1; iz <= 1; iz++) {
+        if (ix == 0 && iy == 0 && iz == 0) continue;
+        double sh[3] = {q[0] + ix * c->box[0], q[1] + iy * c->box[1], q[2] + iz * c->box[2]};
+        double minD2 = min_dist_sq_to_box(c, sh);
+        if (minD2 > merged[k - 1].d + TOL) continue;
+        knn_search(c, sh, k, img_ids, img_r2);
+        for (size_t i = 0; i < k; i++) {
+          int found = 0;
+          for (size_t j = 0; j < m; j++) {
+            if (merged[j].id == img_ids[i]) {
+              merged[j].d = (img_r2[i] < merged[j].d) ? img_r2[i] : merged[j].d;
+              found = 1;
+              break;
+            }
+          }
+          if (!found) {
+            merged[m].id = img_ids[i];
+            merged[m].d = img_r2[i];
+            m++;
+          }
+        }
+        /* std::sort by dist2; for <= 16 elements libstdc++ runs a plain
+         * insertion sort, which is stable.  Larger merges (k > 8) only arise
+         * for the 100-NN classification, which reads set membership only. */
+        for (size_t i = 1; i < m; i++) {
+          iddist v = merged[i];
+          size_t j = i;
+          while (j > 0 && v.d < merged[j - 1].d) {
+            merged[j] = merged[j - 1];
+            j--;
+          }
+          merged[j] = v;
+        }
+        if (m > k) m = k;
+      }
+  for (size_t i = 0; i < k; i++) {
+    res[i] = merged[i].id;
+    r2[i] = merged[i].d;
+  }
+  if (merged != stackbuf) free(merged);
+  free(img_ids);
+  free(img_r2);
+}
+
+static size_t query1(const vto_cloud *c, const double *q) {
+  size_t id;
+  double r2;
+  query_k(c, q, 1, &id, &r2);
+  return id;
+}
+
+/* ---------------------------------------------------------------------------
+ * Ray walk (src/ray.cpp).
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+  size_t tree_id, next;
+  double s;
+} raypt;
+
+typedef struct {
+  size_t cell;
+  double s_enter, s_exit;
+} seg;
+
+typedef struct {
+  double start[3], end[3], dir[3];
+  raypt *pts;
+  size_t npts, cap;
+  seg *segs;
+  size_t nseg, segcap;
+} ray;
+
+static void ray_push_pt(ray *r, size_t id, size_t next, double s) {
+  if (r->npts == r->cap) {
+    r->cap = r->cap ? 2 * r->cap : 64;
+    r->pts = realloc(r->pts, r->cap * sizeof(raypt));
+  }
+  r->pts[r->npts].tree_id = id;
+  r->pts[r->npts].next = next;
+  r->pts[r->npts].s = s;
+  r->npts++;
+}
+
+static void ray_push_seg(ray *r, size_t id, double a, double b) {
+  if (r->nseg == r->segcap) {
+    r->segcap = r->segcap ? 2 * r->segcap : 64;
+    r->segs = realloc(r->segs, r->segcap * sizeof(seg));
+  }
+  r->segs[r->nseg].cell = id;
+  r->segs[r->nseg].s_enter = a;
+  r->segs[r->nseg].s_exit = b;
+  r->nseg++;
+}
+
+static void ray_init(ray *r, const double *a, const double *b) {
+  /* Ray::Ray, src/ray.cpp:14-35 */
+  for (int j = 0; j < 3; j++) {
+    r->start[j] = a[j];
+    r->end[j] = b[j];
+    r->dir[j] = b[j] - a[j];
+  }
+  double s = sqrt(r->dir[0] * r->dir[0] + r->dir[1] * r->dir[1] + r->dir[2] * r->dir[2]);
+  r->dir[0] /= s;
+  r->dir[1] /= s;
+  r->dir[2] /= s;
+  r->npts = 0;
+  r->nseg = 0;
+  ray_push_pt(r, NONE, 1, 0.0);
+  ray_push_pt(r, NONE, NONE, s);
+}
+
+static double split_analytic(const ray *r, const double *p1, const double *p2) {
+  /* findSplitPointDistance(pos1, pos2), src/ray.cpp:38-60 */
+  double norm[3], ppl[3];
+  for (int j = 0; j < 3; j++) norm[j] = p2[j] - p1[j];
+  for (int j = 0; j < 3; j++) ppl[j] = 0.5 * (p1[j] + p2[j]) - r->start[j];
+  double ndp = norm[0] * ppl[0] + norm[1] * ppl[1] + norm[2] * ppl[2];
+  double ndd = norm[0] * r->dir[0] + norm[1] * r->dir[1] + norm[2] * r->dir[2];
+  if (fabs(ndd) < PAR_TOL) return DBL_MAX;
+  return ndp / ndd;
+}
+
+static int split_full(const vto_cloud *c, const ray *r, size_t id1, size_t id2, double s_lo,
+                      double s_hi, double *out) {
+  /* findSplitPointDistance(cloud, id1, id2, s_lo, s_hi), src/ray.cpp:64-127 */
+  double p1[3], p2[3];
+  for (int j = 0; j < 3; j++) {
+    p1[j] = P(c, id1, j);
+    p2[j] = P(c, id2, j);
+  }
+  if (c->periodic) {
+    double s_ref = 0.5 * (s_lo + s_hi);
+    for (int d = 0; d < 3; d++) {
+      double ref = r->start[d] + s_ref * r->dir[d];
+      double diff1 = p1[d] - ref;
+      if (diff1 > c->box[d] / 2)
+        p1[d] -= c->box[d];
+      else if (diff1 < -c->box[d] / 2)
+        p1[d] += c->box[d];
+      double diff2 = p2[d] - ref;
+      if (diff2 > c->box[d] / 2)
+        p2[d] -= c->box[d];
+      else if (diff2 < -c->box[d] / 2)
+        p2[d] += c->box[d];
+    }
+  }
+  double s = split_analytic(r, p1, p2);
+  if (s < DBL_MAX * 0.5 && s >= s_lo - TOL && s <= s_hi + TOL) {
+    *out = s;
+    return 0;
+  }
+  double lo = s_lo, hi = s_hi;
+  for (int it = 0; it < 64; it++) {
+    double mid_s = 0.5 * (lo + hi);
+    double mid[3];
+    for (int j = 0; j < 3; j++) mid[j] = r->start[j] + mid_s * r->dir[j];
+    size_t mc = query1(c, mid);
+    if (mc != id1 && mc != id2) {
+      *out = mid_s;
+      return 0;
+    }
+    if (mc == id1)
+      lo = mid_s;
+    else
+      hi = mid_s;
+    if (hi - lo < 1e-12) {
+      *out = 0.5 * (lo + hi);
+      return 0;
+    }
+  }
+  return set_err(2, "Binary search did not converge for split point between cells %zu and %zu",
+                 id1, id2);
+}
+
+static int perp_perturb(const vto_cloud *c, const ray *r, const double *pos, size_t ex1,
+                        size_t ex2, size_t *out) {
+  /* Ray::perpPerturbToFindCell, src/ray.cpp:129-185 */
+  const double INV_SQRT2 = 1.0 / sqrt(2.0);
+  const double w[3][3] = {{INV_SQRT2, INV_SQRT2, 0.0},
+                          {0.0, INV_SQRT2, INV_SQRT2},
+                          {INV_SQRT2, 0.0, INV_SQRT2}};
+  int cycle = 0;
+  for (int bound = 1; cycle < MAX_CYC; bound++)
+    for (int a = -bound; a <= bound && cycle < MAX_CYC; a++)
+      for (int b = -bound; b <= bound && cycle < MAX_CYC; b++)
+        for (int cc = -bound; cc <= bound && cycle < MAX_CYC; cc++) {
+          if (a == 0 && b == 0 && cc == 0) continue;
+          if (abs(a) < bound && abs(b) < bound && abs(cc) < bound) continue;
+          double pd[3], cr[3];
+          for (int j = 0; j < 3; j++) pd[j] = a * w[0][j] + b * w[1][j] + cc * w[2][j];
+          cr[0] = r->dir[1] * pd[2] - r->dir[2] * pd[1];
+          cr[1] = r->dir[2] * pd[0] - r->dir[0] * pd[2];
+          cr[2] = r->dir[0] * pd[1] - r->dir[1] * pd[0];
+          double mag = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+          if (mag < 1e-15) {
+            cycle++;
+            continue;
+          }
+          for (int j = 0; j < 3; j++) cr[j] /= mag;
+          double pp[3];
+          for (int j = 0; j < 3; j++) pp[j] = pos[j] + PERT_EPS * cr[j];
+          size_t cell = query1(c, pp);
+          if (cell != ex1 && cell != ex2) {
+            *out = cell;
+            return 0;
+          }
+          cycle++;
+        }
+  return set_err(2, "Perturbation did not resolve degenerate split point after %d cycles",
+                 MAX_CYC);
+}
+
+static size_t try_along_ray(const vto_cloud *c, const ray *r, const double *pos, double sign,
+                            size_t ct, size_t nt) {
+  /* the tryAlongRay lambda, src/ray.cpp:374-384 */
+  double pp[3];
+  for (int j = 0; j < 3; j++) pp[j] = pos[j] + sign * PERT_EPS * r->dir[j];
+  size_t ids[2];
+  double r2[2];
+  query_k(c, pp, 2, ids, r2);
+  if (r2[0] + TOL < r2[1] && ids[0] != ct && ids[0] != nt) return ids[0];
+  return NONE;
+}
+
+static int ray_walk(const vto_cloud *c, ray *r) {
+  /* Ray::walk, src/ray.cpp:187-430 */
+  r->nseg = 0;
+  r->npts = 2;
+  r->pts[0].next = 1;
+  r->pts[1].next = NONE;
+  r->pts[0].tree_id = query1(c, r->start);
+  r->pts[1].tree_id = query1(c, r->end);
+  size_t open_cell = r->pts[0].tree_id;
+  double entry_s = r->pts[0].s;
+
+  if (r->pts[0].tree_id == r->pts[1].tree_id) {
+    if (!c->periodic) {
+      ray_push_seg(r, r->pts[0].tree_id, r->pts[0].s, r->pts[1].s);
+      return 0;
+    }
+    size_t start_cell = r->pts[0].tree_id;
+    double total_s = r->pts[1].s;
+    int found = 0;
+    for (int level = 1; level <= 10 && !found; level++) {
+      int n = 1 << level;
+      for (int i = 1; i < n && !found; i += 2) {
+        double mid_s = total_s * i / n;
+        double mid[3];
+        for (int j = 0; j < 3; j++) mid[j] = r->start[j] + mid_s * r->dir[j];
+        size_t mc = query1(c, mid);
+        if (mc != start_cell) {
+          ray_push_pt(r, mc, 1, mid_s);
+          r->pts[0].next = r->npts - 1;
+          found = 1;
+        }
+      }
+    }
+    if (!found)
+      return set_err(2, "Could not find intermediate cell by bisection along periodic ray");
+  }
+
+  size_t current = 0;
+  size_t ct = r->pts[current].tree_id;
+  size_t next = r->pts[current].next;
+  size_t nt = r->pts[next].tree_id;
+  size_t iteration = 0;
+  int not_done = 1;
+  double s, pos[3];
+  while (not_done) {
+    if (++iteration > MAX_IT) return set_err(2, "Ray walk exceeded maximum iteration count");
+    if (r->pts[next].s - r->pts[current].s < 1e-14) {
+      current = next;
+      ct = r->pts[current].tree_id;
+      open_cell = ct;
+      next = r->pts[current].next;
+      if (next == NONE)
+        not_done = 0;
+      else
+        nt = r->pts[next].tree_id;
+      continue;
+    }
+    int rc = split_full(c, r, ct, nt, r->pts[current].s, r->pts[next].s, &s);
+    if (rc) return rc;
+    if (s < r->pts[current].s - TOL || s > r->pts[next].s + TOL)
+      return set_err(2,
+                     "Split point outside segment bounds: s=%f not in (%f, %f) for cells %zu "
+                     "and %zu",
+                     s, r->pts[current].s, r->pts[next].s, ct, nt);
+    {
+      double lo = r->pts[current].s + 1e-15, hi = r->pts[next].s - 1e-15;
+      s = (s < lo) ? lo : ((hi < s) ? hi : s); /* std::clamp */
+    }
+    for (int j = 0; j < 3; j++) pos[j] = r->start[j] + s * r->dir[j];
+
+    size_t ids[3];
+    double r2v[3];
+    query_k(c, pos, 3, ids, r2v);
+    int has_c = 0, has_n = 0, three_eq = 1;
+    for (int i = 0; i < 3; i++) {
+      if (r2v[i] - r2v[0] >= TOL) {
+        three_eq = 0;
+        break;
+      }
+      if (ids[i] == ct) has_c = 1;
+      if (ids[i] == nt) has_n = 1;
+    }
+    if (three_eq && !has_c && !has_n) {
+      size_t kb = 100 < c->n ? 100 : c->n;
+      size_t idb[100];
+      double r2b[100];
+      query_k(c, pos, kb, idb, r2b);
+      for (size_t i = 0; i < kb; i++) {
+        if (r2b[i] - r2b[0] >= TOL) break;
+        if (idb[i] == ct) has_c = 1;
+        if (idb[i] == nt) has_n = 1;
+        if (i == kb - 1 && !has_c && !has_n)
+          return set_err(2,
+                         "Could not resolve split point classification after expanding to "
+                         "%d-NN. This should be very unlikely; consider increasing TOLERANCE "
+                         "or K_EXPAND.",
+                         100);
+      }
+    }
+    if (has_c || has_n) {
+      ray_push_seg(r, open_cell, entry_s, s);
+      open_cell = nt;
+      entry_s = s;
+      current = r->pts[current].next;
+      ct = r->pts[current].tree_id;
+      next = r->pts[current].next;
+      if (next == NONE)
+        not_done = 0;
+      else
+        nt = r->pts[next].tree_id;
+    } else if (r2v[0] + TOL < r2v[1]) {
+      ray_push_pt(r, ids[0], next, s);
+      next = r->npts - 1;
+      r->pts[current].next = next;
+      nt = ids[0];
+    } else {
+      size_t sc = try_along_ray(c, r, pos, -1.0, ct, nt);
+      if (sc == NONE) {
+        double pc[3];
+        for (int j = 0; j < 3; j++) pc[j] = pos[j] - PERT_EPS * r->dir[j];
+        rc = perp_perturb(c, r, pc, ct, nt, &sc);
+        if (rc) return rc;
+      }
+      size_t sn = try_along_ray(c, r, pos, 1.0, ct, nt);
+      if (sn == NONE) {
+        double pn[3];
+        for (int j = 0; j < 3; j++) pn[j] = pos[j] + PERT_EPS * r->dir[j];
+        rc = perp_perturb(c, r, pn, ct, nt, &sn);
+        if (rc) return rc;
+      }
+      if (sc == sn) {
+        ray_push_pt(r, sc, next, s);
+        next = r->npts - 1;
+        r->pts[current].next = next;
+        nt = sc;
+      } else {
+        size_t sn_idx = r->npts;
+        ray_push_pt(r, sn, next, s);
+        size_t sc_idx = r->npts;
+        ray_push_pt(r, sc, sn_idx, s);
+        r->pts[current].next = sc_idx;
+        next = sc_idx;
+        nt = sc;
+      }
+    }
+  }
+  ray_push_seg(r, open_cell, entry_s, r->pts[current].s);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Reductions (src/reduction.cpp:8-101).
+ * ------------------------------------------------------------------------- */
+
+static size_t n_out_for(const vto_cloud *c, int mode) { return mode == 3 ? 3 : c->nf; }
+
+static int reduce(const vto_cloud *c, const ray *r, int mode, double *out) {
+  size_t nf = c->nf;
+  switch (mode) {
+    case 0:
+      for (size_t f = 0; f < nf; f++) out[f] = 0.0;
+      for (size_t i = 0; i < r->nseg; i++)
+        for (size_t f = 0; f < nf; f++)
+          out[f] += (r->segs[i].s_exit - r->segs[i].s_enter) * c->fields[r->segs[i].cell * nf + f];
+      return 0;
+    case 1:
+    case 2:
+      for (size_t f = 0; f < nf; f++) out[f] = mode == 1 ? -INFINITY : INFINITY;
+      for (size_t i = 0; i < r->nseg; i++)
+        for (size_t f = 0; f < nf; f++) {
+          double v = c->fields[r->segs[i].cell * nf + f];
+          if (mode == 1 ? (v > out[f]) : (v < out[f])) out[f] = v;
+        }
+      return 0;
+    case 3: {
+      if (nf != 4)
+        return set_err(1, "VolumeRender requires exactly 4 fields (R, G, B, alpha), got %zu", nf);
+      if (!c->valid_rgba)
+        return set_err(1, "Volume rendering requires R, G, B values in [0, 1] and alpha >= 0");
+      double T = 1.0;
+      out[0] = out[1] = out[2] = 0.0;
+      for (size_t i = 0; i < r->nseg; i++) {
+        const double *fv = c->fields + r->segs[i].cell * 4;
+        double alpha = fv[3];
+        double ds = r->segs[i].s_exit - r->segs[i].s_enter;
+        double opacity = 1.0 - exp(-alpha * ds);
+        for (int k = 0; k < 3; k++) out[k] += T * fv[k] * opacity;
+        T *= (1.0 - opacity);
+      }
+      return 0;
+    }
+  }
+  return set_err(1, "Unrecognized reduction mode");
+}
+
+/* ---------------------------------------------------------------------------
+ * Cloud construction (src/pointcloud.cpp:12-212).
+ * ------------------------------------------------------------------------- */
+
+int vto_cloud_create(const double *pos, size_t npart, const double *fields, size_t nfields,
+                     const double subbox[6], const double *vol, size_t nvol, int periodic,
+                     int build, vto_cloud **out) {
+  vto_cloud *c = calloc(1, sizeof *c);
+  c->nf = nfields;
+  memcpy(c->subbox, subbox, 6 * sizeof(double));
+  c->periodic = periodic;
+  if (periodic) {
+    c->box[0] = subbox[1] - subbox[0];
+    c->box[1] = subbox[3] - subbox[2];
+    c->box[2] = subbox[5] - subbox[4];
+  }
+  if (nvol > 0) {
+    double min_vol = DBL_MAX;
+    for (size_t i = 0; i < nvol; i++)
+      if (vol[i] > 0 && vol[i] < min_vol) min_vol = vol[i];
+    if (min_vol < DBL_MAX) {
+      double min_radius = cbrt(3.0 * min_vol / (4.0 * M_PI));
+      if (min_radius < 1e-6)
+        add_warning(c,
+                    "Some cells have radii smaller than 1e-6, which approaches the internal "
+                    "ray-tracing tolerance (1e-9). Results may be unreliable for very small "
+                    "cells. Consider rescaling your coordinate system.");
+    }
+  }
+  c->pts = malloc((npart ? npart : 1) * 3 * sizeof(double));
+  c->fields = malloc((npart ? npart : 1) * (nfields ? nfields : 1) * sizeof(double));
+  c->orig_ids = malloc((npart ? npart : 1) * sizeof(int64_t));
+  size_t n = 0;
+  if (!periodic) {
+    double dx = subbox[1] - subbox[0], dy = subbox[3] - subbox[2], dz = subbox[5] - subbox[4];
+    if (nvol > 0) {
+      double max_vol = 0.0;
+      for (size_t i = 0; i < nvol; i++)
+        if (vol[i] > max_vol) max_vol = vol[i];
+      c->pad = 10.0 * cbrt(3.0 * max_vol / (4.0 * M_PI));
+    } else {
+      /* std::max({dx, dy, dz}) keeps the first of equal maxima */
+      double m = dx;
+      if (m < dy) m = dy;
+      if (m < dz) m = dz;
+      c->pad = 0.15 * m;
+      add_warning(c,
+                  "No cell volumes provided; using 15% of longest box side as padding. Pass "
+                  "vol= for adaptive padding.");
+    }
+    double lo[3] = {subbox[0] - c->pad, subbox[2] - c->pad, subbox[4] - c->pad};
+    double hi[3] = {subbox[1] + c->pad, subbox[3] + c->pad, subbox[5] + c->pad};
+    for (size_t i = 0; i < npart; i++) {
+      const double *p = pos + 3 * i;
+      if (p[0] >= lo[0] && p[0] <= hi[0] && p[1] >= lo[1] && p[1] <= hi[1] && p[2] >= lo[2] &&
+          p[2] <= hi[2]) {
+        memcpy(c->pts + 3 * n, p, 3 * sizeof(double));
+        memcpy(c->fields + n * nfields, fields + i * nfields, nfields * sizeof(double));
+        c->orig_ids[n] = (int64_t)i;
+        n++;
+      }
+    }
+  } else {
+    double pmin[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, pmax[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (size_t i = 0; i < npart; i++) {
+      memcpy(c->pts + 3 * i, pos + 3 * i, 3 * sizeof(double));
+      for (int d = 0; d < 3; d++) {
+        if (pos[3 * i + d] < pmin[d]) pmin[d] = pos[3 * i + d];
+        if (pos[3 * i + d] > pmax[d]) pmax[d] = pos[3 * i + d];
+      }
+      memcpy(c->fields + i * nfields, fields + i * nfields, nfields * sizeof(double));
+      c->orig_ids[i] = (int64_t)i;
+    }
+    n = npart;
+    if (pmin[0] < subbox[0] || pmax[0] > subbox[1] || pmin[1] < subbox[2] ||
+        pmax[1] > subbox[3] || pmin[2] < subbox[4] || pmax[2] > subbox[5]) {
+      vto_cloud_destroy(c);
+      return set_err(1,
+                     "Periodic mode: some particles lie outside the bounding box. All "
+                     "particles must be within the periodic domain.");
+    }
+    for (int d = 0; d < 3; d++)
+      if (pmax[d] - pmin[d] < 0.6 * c->box[d]) {
+        add_warning(c,
+                    "Periodic mode: particle extent is less than 60% of the bounding box in "
+                    "at least one dimension. The bounding box may be too large for the "
+                    "particle distribution.");
+        break;
+      }
+  }
+  c->n = n;
+  c->valid_rgba = 0;
+  if (nfields == 4 && n > 0) {
+    c->valid_rgba = 1;
+    for (size_t i = 0; i < n && c->valid_rgba; i++) {
+      for (int k = 0; k < 3; k++) {
+        double v = c->fields[i * 4 + k];
+        if (v < 0.0 || v > 1.0) {
+          c->valid_rgba = 0;
+          break;
+        }
+      }
+      if (!c->valid_rgba) break;
+      if (c->fields[i * 4 + 3] < 0.0) c->valid_rgba = 0;
+    }
+  }
+  if (build) {
+    if (n == 0) {
+      vto_cloud_destroy(c);
+      return set_err(2, "There are no points in the cloud; cannot build tree");
+    }
+    build_tree(c);
+  }
+  *out = c;
+  return 0;
+}
+
+void vto_cloud_destroy(vto_cloud *c) {
+  if (!c) return;
+  free(c->pts);
+  free(c->fields);
+  free(c->orig_ids);
+  free(c->vind);
+  free(c->nodes);
+  free(c);
+}
+
+size_t vto_cloud_npart(const vto_cloud *c) { return c->n; }
+int vto_cloud_valid_rgba(const vto_cloud *c) { return c->valid_rgba; }
+double vto_cloud_pad(const vto_cloud *c) { return c->pad; }
+void vto_cloud_orig_ids(const vto_cloud *c, int64_t *out) {
+  memcpy(out, c->orig_ids, c->n * sizeof(int64_t));
+}
+int vto_cloud_num_warnings(const vto_cloud *c) { return c->nwarn; }
+const char *vto_cloud_warning(const vto_cloud *c, int i) {
+  return (i >= 0 && i < c->nwarn) ? c->warn[i] : "";
+}
+
+size_t vto_tree_nnodes(const vto_cloud *c) { return c->nnodes; }
+
+void vto_tree_export(const vto_cloud *c, int64_t *vind, int32_t *feat, int64_t *a, int64_t *b,
+                     double *divlow, double *divhigh, double *root_bbox6) {
+  for (size_t i = 0; i < c->n; i++) vind[i] = (int64_t)c->vind[i];
+  for (size_t i = 0; i < c->nnodes; i++) {
+    const onode *nd = &c->nodes[i];
+    feat[i] = nd->feat;
+    a[i] = nd->feat < 0 ? (int64_t)nd->left : nd->c1;
+    b[i] = nd->feat < 0 ? (int64_t)nd->right : nd->c2;
+    divlow[i] = nd->feat < 0 ? 0.0 : nd->lo;
+    divhigh[i] = nd->feat < 0 ? 0.0 : nd->hi;
+  }
+  for (int d = 0; d < 3; d++) {
+    root_bbox6[2 * d] = c->root_bbox[d][0];
+    root_bbox6[2 * d + 1] = c->root_bbox[d][1];
+  }
+}
+
+int vto_query(const vto_cloud *c, const double *q, size_t nq, size_t k, int64_t *ids,
+              double *r2) {
+  if (!c->tree_built) return set_err(2, "There is currently no valid tree for this point cloud");
+  size_t *tmp = malloc(k * sizeof(size_t));
+  for (size_t i = 0; i < nq; i++) {
+    query_k(c, q + 3 * i, k, tmp, r2 + i * k);
+    for (size_t j = 0; j < k; j++) ids[i * k + j] = tmp[j] == NONE ? -1 : (int64_t)tmp[j];
+  }
+  free(tmp);
+  return 0;
+}
+
+struct vto_segments {
+  size_t nrays, nout, nseg;
+  double *values;
+  int64_t *offsets;
+  seg *segs;
+};
+
+static int run_rays(const vto_cloud *c, const double *start, const double *end, size_t nrays,
+                    int mode, double *out, int nthreads, seg **segs_out, size_t *counts) {
+  if (!c->tree_built) return set_err(2, "There is currently no valid tree for this point cloud");
+  size_t nout = n_out_for(c, mode);
+  int err = 0;
+  char errmsg[512] = {0};
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+  {
+    ray r;
+    memset(&r, 0, sizeof r);
+    double zero[3] = {0, 0, 0};
+    ray_init(&r, zero, zero);
+#pragma omp for schedule(dynamic, 256)
+    for (size_t i = 0; i < nrays; i++) {
+      int rc;
+#pragma omp atomic read
+      rc = err;
+      if (rc) continue;
+      ray_init(&r, start + 3 * i, end + 3 * i);
+      rc = ray_walk(c, &r);
+      if (!rc) rc = reduce(c, &r, mode, out + i * nout);
+      if (!rc && segs_out) {
+        segs_out[i] = malloc((r.nseg ? r.nseg : 1) * sizeof(seg));
+        memcpy(segs_out[i], r.segs, r.nseg * sizeof(seg));
+        counts[i] = r.nseg;
+      }
+      if (rc) {
+#pragma omp critical
+        {
+          if (!err) {
+            err = rc;
+            snprintf(errmsg, sizeof errmsg, "%s", g_err);
+          }
+        }
+      }
+    }
+    free(r.pts);
+    free(r.segs);
+  }
+  if (err) return set_err(err, "%s", errmsg);
+  return 0;
+}
+
+int vto_project(const vto_cloud *c, const double *start, const double *end, size_t nrays,
+                int mode, double *out, int nthreads) {
+  if (mode < 0 || mode > 3) return set_err(1, "Unrecognized reduction mode");
+  return run_rays(c, start, end, nrays, mode, out, nthreads, NULL, NULL);
+}
+
+int vto_detailed(const vto_cloud *c, const double *start, const double *end, size_t nrays,
+                 int mode, int nthreads, vto_segments **out, size_t *nseg) {
+  if (mode < 0 || mode > 3) return set_err(1, "Unrecognized reduction mode");
+  vto_segments *s = calloc(1, sizeof *s);
+  s->nrays = nrays;
+  s->nout = n_out_for(c, mode);
+  s->values = calloc(nrays * s->nout + 1, sizeof(double));
+  seg **per = calloc(nrays + 1, sizeof(seg *));
+  size_t *counts = calloc(nrays + 1, sizeof(size_t));
+  int rc = run_rays(c, start, end, nrays, mode, s->values, nthreads, per, counts);
+  if (!rc) {
+    s->offsets = malloc((nrays + 1) * sizeof(int64_t));
+    s->offsets[0] = 0;
+    for (size_t i = 0; i < nrays; i++) s->offsets[i + 1] = s->offsets[i] + (int64_t)counts[i];
+    s->nseg = (size_t)s->offsets[nrays];
+    s->segs = malloc((s->nseg ? s->nseg : 1) * sizeof(seg));
+    for (size_t i = 0; i < nrays; i++)
+      memcpy(s->segs + s->offsets[i], per[i], counts[i] * sizeof(seg));
+  }
+  for (size_t i = 0; i < nrays; i++) free(per[i]);
+  free(per);
+  free(counts);
+  if (rc) {
+    vto_detailed_destroy(s);
+    return rc;
+  }
+  *out = s;
+  *nseg = s->nseg;
+  return 0;
+}
+
+void vto_detailed_copy(const vto_segments *s, double *values, int64_t *cell_ids,
+                       double *s_enter, double *s_exit, int64_t *offsets) {
+  memcpy(values, s->values, s->nrays * s->nout * sizeof(double));
+  for (size_t i = 0; i < s->nseg; i++) {
+    cell_ids[i] = (int64_t)s->segs[i].cell;
+    s_enter[i] = s->segs[i].s_enter;
+    s_exit[i] = s->segs[i].s_exit;
+  }
+  memcpy(offsets, s->offsets, (s->nrays + 1) * sizeof(int64_t));
+}
+
+void vto_detailed_destroy(vto_segments *s) {
+  if (!s) return;
+  free(s->values);
+  free(s->offsets);
+  free(s->segs);
+  free(s);
+}
