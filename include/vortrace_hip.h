@@ -1,0 +1,132 @@
+/* vortrace_hip.h -- C-ABI of the MI355X (gfx950) ray-projection library.
+ *
+ * This is the drop-in boundary that replaces the reference's pybind11 module
+ * `Cvortrace` (src/bindings.cpp:71-326).  Every entry point takes plain
+ * pointers and sizes; arrays marked d_* live in device memory on the device
+ * that was current when the cloud was built, all others in host memory.
+ * Calls are ordered on the HIP stream passed as `stream` (a hipStream_t, or
+ * NULL for the default stream) and return after the work has completed, so
+ * that errors raised inside kernels can be reported as status codes.
+ *
+ * Status codes: VT_OK, VT_EINVAL (the reference throws std::invalid_argument,
+ * which its binding maps to ValueError), VT_ERUNTIME (any other std::exception,
+ * RuntimeError).  vt_last_error() returns the message of the last failure on
+ * the calling thread, with the reference's wording.
+ *
+ * Cell ids crossing the boundary are indices into the filtered particle list,
+ * as in the reference (PointCloud::orig_ids maps them to input rows).
+ */
+#ifndef VORTRACE_HIP_H
+#define VORTRACE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { VT_OK = 0, VT_EINVAL = 1, VT_ERUNTIME = 2 };
+
+/* ReductionMode, include/mytypes.hpp:14 (values are part of the ABI). */
+enum { VT_SUM = 0, VT_MAX = 1, VT_MIN = 2, VT_VOLUME = 3 };
+
+typedef struct vt_cloud vt_cloud;
+
+/* Message of the last failed call on this thread (never NULL). */
+const char *vt_last_error(void);
+
+/* vortrace::warn (include/mytypes.hpp:19-25) routed to a callback; the
+ * default prints "vortrace warning: <msg>" to stderr.  NULL restores it. */
+typedef void (*vt_warning_fn)(const char *msg, void *user);
+void vt_set_warning_callback(vt_warning_fn fn, void *user);
+
+/* set_verbose / get_verbose (src/bindings.cpp:91-95). */
+void vt_set_verbose(int verbose);
+int vt_get_verbose(void);
+
+/* PointCloud::loadPoints (src/pointcloud.cpp:12-191; binding
+ * src/bindings.cpp:29-69,112-116).  pos is npart x 3, fields npart x nfields,
+ * both row-major FP64; subbox = {xmin,xmax,ymin,ymax,zmin,zmax}; vol may be
+ * NULL (nvol = 0).  Filters to the padded box (non-periodic) or validates
+ * the periodic domain, and computes the RGBA validity flag. */
+int vt_cloud_create(const double *pos, size_t npart, const double *fields, size_t nfields,
+                    const double subbox[6], const double *vol, size_t nvol, int periodic,
+                    vt_cloud **out);
+
+/* PointCloud::buildTree (src/pointcloud.cpp:193-212): the nanoflann 1.3.2
+ * L=10 static kd-tree, with identical split decisions, flattened into the
+ * device layout and uploaded to the current HIP device. */
+int vt_cloud_build_tree(vt_cloud *cloud);
+
+/* The two halves of vt_cloud_build_tree: the host-side build (touches no
+ * device) and the upload of a built tree to the current HIP device. */
+int vt_cloud_build_host_tree(vt_cloud *cloud);
+int vt_cloud_upload(vt_cloud *cloud);
+
+/* Copies the host tree out in pre-order (no reference counterpart; used to
+ * check the build against nanoflann's index dump, include/nanoflann.hpp:
+ * 1023-1071).  vind has npart entries; the node arrays have
+ * vt_tree_info.nodes entries: feat = -1 for a leaf with a/b = first/end
+ * position in vind, else the cut dimension with a/b = child1/child2 node
+ * indices and divlow/divhigh; bbox6 = root box {xlo,xhi,ylo,yhi,zlo,zhi}. */
+int vt_cloud_tree_export(const vt_cloud *cloud, int64_t *vind, int32_t *feat, int64_t *a,
+                         int64_t *b, double *divlow, double *divhigh, double *bbox6);
+
+void vt_cloud_destroy(vt_cloud *cloud);
+
+/* Getters (src/bindings.cpp:134-154). */
+size_t vt_cloud_npart(const vt_cloud *cloud);
+size_t vt_cloud_nfields(const vt_cloud *cloud);
+int vt_cloud_periodic(const vt_cloud *cloud);
+int vt_cloud_valid_rgba(const vt_cloud *cloud);
+int vt_cloud_tree_built(const vt_cloud *cloud);
+double vt_cloud_pad(const vt_cloud *cloud);
+void vt_cloud_subbox(const vt_cloud *cloud, double out6[6]);
+/* out has vt_cloud_npart() entries. */
+void vt_cloud_orig_ids(const vt_cloud *cloud, int64_t *out);
+
+/* Tree statistics: number of nodes, leaves and the maximum depth (root = 0). */
+typedef struct {
+  int64_t nodes, leaves, max_depth;
+  double build_seconds;
+} vt_tree_info;
+int vt_cloud_tree_info(const vt_cloud *cloud, vt_tree_info *out);
+
+/* PointCloud::queryTreeK (src/pointcloud.cpp:308-368, binding 155-174) for nq
+ * device-resident query points: ids/r2 are nq x k, sorted by r^2 with
+ * nanoflann's tie order.  1 <= k <= 128.  Test hook and building block. */
+int vt_query_knn(const vt_cloud *cloud, const double *d_q, size_t nq, int k, int64_t *d_ids,
+                 double *d_r2, void *stream);
+
+/* Projection::makeProjection (src/projection.cpp:26-73): one ray per row of
+ * d_start/d_end (nrays x 3), result d_out nrays x n_out with n_out = 3 for
+ * VT_VOLUME and nfields otherwise (src/reduction.cpp:98-101). */
+int vt_project(const vt_cloud *cloud, const double *d_start, const double *d_end, size_t nrays,
+               int mode, double *d_out, void *stream);
+
+/* Projection::makeDetailedProjection (src/projection.cpp:75-144) in two
+ * passes.  vt_project_count writes the number of segments of every ray to
+ * d_counts (nrays); the caller forms offsets (exclusive scan, nrays + 1) and
+ * calls vt_project_segments, which writes the reduced values and, at
+ * d_offsets[i], ray i's ordered segments: cell id, entry and exit distance
+ * (returnSegments, src/bindings.cpp:206-238). */
+int vt_project_count(const vt_cloud *cloud, const double *d_start, const double *d_end,
+                     size_t nrays, int64_t *d_counts, void *stream);
+int vt_project_segments(const vt_cloud *cloud, const double *d_start, const double *d_end,
+                        size_t nrays, int mode, const int64_t *d_offsets, double *d_out,
+                        int64_t *d_cell_ids, double *d_s_enter, double *d_s_exit,
+                        void *stream);
+
+/* Per-call statistics of the last vt_project* call on this thread: total
+ * segments (cell crossings), kNN queries and the deepest walk stack. */
+typedef struct {
+  int64_t segments, queries, max_walk_depth, max_knn_depth, slow_queries;
+  float kernel_ms;
+} vt_call_stats;
+int vt_last_call_stats(vt_call_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,298 @@
+"""Low-level classes mirroring the reference's ``Cvortrace`` module.
+
+Same class, method and argument names as src/bindings.cpp:71-326
+(``PointCloud``, ``Projection``, ``ReductionMode``, ``set_verbose``,
+``get_verbose``) over the C-ABI of include/vortrace_hip.h.  Inputs are numpy
+arrays or torch tensors; ray and query arrays go to the device the cloud was
+built on; ``return*`` methods give numpy arrays like the reference.  The
+``*_tensor`` methods keep everything on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_P = C.c_void_p
+
+
+class ReductionMode(enum.IntEnum):
+    """include/mytypes.hpp:14 (values are part of the ABI)."""
+    Sum = 0
+    Max = 1
+    Min = 2
+    VolumeRender = 3
+
+
+Sum, Max, Min, VolumeRender = (ReductionMode.Sum, ReductionMode.Max,
+                               ReductionMode.Min, ReductionMode.VolumeRender)
+
+
+def set_verbose(verbose: bool) -> None:
+    _lib.lib().vt_set_verbose(int(bool(verbose)))
+
+
+def get_verbose() -> bool:
+    return bool(_lib.lib().vt_get_verbose())
+
+
+def _host_f64(a, name):
+    if isinstance(a, torch.Tensor):
+        a = a.detach().cpu().numpy()
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_P)
+
+
+def _stream():
+    return _P(torch.cuda.current_stream().cuda_stream)
+
+
+def device_f64(a, device) -> torch.Tensor:
+    """A contiguous FP64 tensor on `device` (copied only when needed)."""
+    if isinstance(a, torch.Tensor):
+        return a.detach().to(device=device, dtype=torch.float64).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(device)
+
+
+class PointCloud:
+    """Particle positions, fields and the device kd-tree (src/pointcloud.cpp)."""
+
+    def __init__(self):
+        self._h = None
+        self._device = None
+        _lib.lib()
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h is not None and _lib._lib is not None:
+            _lib._lib.vt_cloud_destroy(h)
+
+    def _handle(self):
+        if self._h is None:
+            raise RuntimeError("PointCloud has no points loaded")
+        return self._h
+
+    # -- construction --------------------------------------------------------
+    def loadPoints(self, pos, fields_in, subbox, vol=None, periodic=False):
+        """PointCloud.loadPoints (src/bindings.cpp:29-69,112-116)."""
+        pos = _host_f64(pos, "pos")
+        fields = _host_f64(fields_in, "fields_in")
+        if pos.ndim != 2:
+            raise ValueError("pos array must be two-dimensional")
+        if fields.ndim == 1:
+            nf, nfp = 1, fields.shape[0]
+        elif fields.ndim == 2:
+            nf, nfp = fields.shape[1], fields.shape[0]
+        else:
+            raise ValueError("fields array must be one-dimensional or two-dimensional")
+        if pos.shape[0] != nfp:
+            raise ValueError(f"Input sizes must match: pos has {pos.shape[0]} "
+                             f"rows but fields has {nfp}")
+        if pos.shape[1] != 3:
+            raise ValueError("pos must have shape (N, 3)")
+        sb = np.ascontiguousarray(np.asarray(subbox, dtype=np.float64).ravel())
+        if sb.size != 6:
+            raise ValueError("subbox must have 6 entries")
+        v = (np.ascontiguousarray(_host_f64(vol, "vol").ravel())
+             if vol is not None else np.zeros(0))
+        h = _P()
+        L = _lib.lib()
+        _lib.check(L.vt_cloud_create(_ptr(pos), pos.shape[0], _ptr(fields), nf,
+                                     _ptr(sb), _ptr(v) if v.size else None,
+                                     v.size, int(bool(periodic)), C.byref(h)))
+        old, self._h = self._h, h
+        if old is not None:
+            L.vt_cloud_destroy(old)
+        self._pos = pos
+        self._fields = fields.reshape(pos.shape[0], nf)
+
+    def buildTree(self):
+        """PointCloud.buildTree: host build, then upload to the current device."""
+        torch.cuda.current_device()  # initialise torch's HIP context first
+        _lib.check(_lib.lib().vt_cloud_build_tree(self._handle()))
+        self._device = torch.device("cuda", torch.cuda.current_device())
+
+    def build_host_tree(self):
+        """Host half of buildTree only (no device needed)."""
+        _lib.check(_lib.lib().vt_cloud_build_host_tree(self._handle()))
+
+    # -- getters (src/bindings.cpp:134-154) -------------------------------------
+    def get_nfields(self) -> int:
+        return int(_lib.lib().vt_cloud_nfields(self._handle()))
+
+    def get_npart(self) -> int:
+        return int(_lib.lib().vt_cloud_npart(self._handle()))
+
+    def get_periodic(self) -> bool:
+        return bool(_lib.lib().vt_cloud_periodic(self._handle()))
+
+    def get_valid_rgba(self) -> bool:
+        return bool(_lib.lib().vt_cloud_valid_rgba(self._handle()))
+
+    def get_tree_built(self) -> bool:
+        return bool(_lib.lib().vt_cloud_tree_built(self._handle()))
+
+    def get_pad(self) -> float:
+        return float(_lib.lib().vt_cloud_pad(self._handle()))
+
+    def get_subbox(self):
+        out = np.zeros(6)
+        _lib.lib().vt_cloud_subbox(self._handle(), _ptr(out))
+        return [float(x) for x in out]
+
+    def get_orig_ids(self) -> np.ndarray:
+        out = np.zeros(self.get_npart(), dtype=np.int64)
+        _lib.lib().vt_cloud_orig_ids(self._handle(), _ptr(out))
+        return out
+
+    def get_pt(self, idx: int):
+        return [float(x) for x in self._pos[self.get_orig_ids()[idx]]]
+
+    def get_field(self, idx: int, f: int) -> float:
+        return float(self._fields[self.get_orig_ids()[idx], f])
+
+    def tree_info(self) -> dict:
+        ti = _lib.TreeInfo()
+        _lib.check(_lib.lib().vt_cloud_tree_info(self._handle(), C.byref(ti)))
+        return {"nodes": ti.nodes, "leaves": ti.leaves,
+                "max_depth": ti.max_depth, "build_seconds": ti.build_seconds}
+
+    def tree_export(self):
+        """(vind, root_bbox (3,2), nodes) in pre-order; nodes are
+        ('leaf', left, right) or ('inner', divfeat, divlow, divhigh)."""
+        ti = self.tree_info()
+        n, nn = self.get_npart(), ti["nodes"]
+        vind = np.zeros(n, np.int64)
+        feat = np.zeros(nn, np.int32)
+        a = np.zeros(nn, np.int64)
+        b = np.zeros(nn, np.int64)
+        lo = np.zeros(nn)
+        hi = np.zeros(nn)
+        bb = np.zeros(6)
+        _lib.check(_lib.lib().vt_cloud_tree_export(
+            self._handle(), _ptr(vind), _ptr(feat), _ptr(a), _ptr(b), _ptr(lo),
+            _ptr(hi), _ptr(bb)))
+        nodes = [("leaf", int(a[i]), int(b[i])) if feat[i] < 0 else
+                 ("inner", int(feat[i]), float(lo[i]), float(hi[i]))
+                 for i in range(nn)]
+        return vind, bb.reshape(3, 2), nodes
+
+    # -- queries -----------------------------------------------------------------
+    def query_tensor(self, q: torch.Tensor, k: int):
+        """k-NN of device-resident queries (nq, 3) -> (ids int64, r2 f64)."""
+        if not self.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        q = device_f64(q, self._device).reshape(-1, 3)
+        ids = torch.empty((q.shape[0], k), dtype=torch.int64, device=self._device)
+        r2 = torch.empty((q.shape[0], k), dtype=torch.float64, device=self._device)
+        _lib.check(_lib.lib().vt_query_knn(self._handle(), _P(q.data_ptr()),
+                                           q.shape[0], int(k), _P(ids.data_ptr()),
+                                           _P(r2.data_ptr()), _stream()))
+        return ids, r2
+
+    def queryTree(self, query_pt, k=None):
+        """PointCloud.queryTree (src/bindings.cpp:155-174): the nearest id, or
+        (ids uint64[k], r2 f64[k]) when k is given."""
+        q = np.asarray(query_pt, dtype=np.float64).reshape(1, 3)
+        ids, r2 = self.query_tensor(q, 1 if k is None else int(k))
+        ids = ids.cpu().numpy()[0]
+        if k is None:
+            return int(ids[0])
+        return ids.astype(np.uint64), r2.cpu().numpy()[0]
+
+
+class Projection:
+    """Rays for one projection (src/projection.cpp, binding 176-238)."""
+
+    def __init__(self, pos_start, pos_end):
+        if not isinstance(pos_start, torch.Tensor):
+            pos_start = np.asarray(pos_start, dtype=np.float64)
+        if not isinstance(pos_end, torch.Tensor):
+            pos_end = np.asarray(pos_end, dtype=np.float64)
+        if pos_start.ndim != 2 or pos_end.ndim != 2:
+            raise ValueError("PROJECTION: pos_start and pos_end array must both be "
+                             "two-dimensional\n")
+        ns = int(np.prod(tuple(pos_start.shape)))
+        ne = int(np.prod(tuple(pos_end.shape)))
+        if ns != ne:
+            raise ValueError("PROJECTION: pos_start and pos_end must have same sizes")
+        self._start = pos_start
+        self._end = pos_end
+        self.ngrid = ns // 3
+        self._data = None
+        self._segments = None
+
+    def _rays(self, cloud: PointCloud):
+        dev = cloud._device
+        s = device_f64(self._start, dev).reshape(-1)[: 3 * self.ngrid].reshape(-1, 3)
+        e = device_f64(self._end, dev).reshape(-1)[: 3 * self.ngrid].reshape(-1, 3)
+        return s.contiguous(), e.contiguous()
+
+    @staticmethod
+    def _nout(cloud, mode):
+        return 3 if int(mode) == VolumeRender else cloud.get_nfields()
+
+    def project_tensor(self, cloud: PointCloud, reduction=Sum) -> torch.Tensor:
+        """makeProjection with the result left on the device (ngrid, n_out)."""
+        if not cloud.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        s, e = self._rays(cloud)
+        out = torch.zeros((self.ngrid, self._nout(cloud, reduction)),
+                          dtype=torch.float64, device=cloud._device)
+        _lib.check(_lib.lib().vt_project(cloud._handle(), _P(s.data_ptr()),
+                                         _P(e.data_ptr()), self.ngrid,
+                                         int(reduction), _P(out.data_ptr()),
+                                         _stream()))
+        return out
+
+    def makeProjection(self, cloud: PointCloud, reduction=Sum):
+        self._data = self.project_tensor(cloud, reduction)
+
+    def returnProjection(self) -> np.ndarray:
+        if self._data is None:
+            raise RuntimeError("Projection has not yet been made")
+        d = self._data.cpu().numpy()
+        return d[:, 0].copy() if d.shape[1] == 1 else d
+
+    def segments_tensor(self, cloud: PointCloud, reduction=Sum):
+        """makeDetailedProjection on the device: (values, cell_ids, s_enter,
+        s_exit, offsets) as device tensors (ids are filtered indices)."""
+        if not cloud.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        s, e = self._rays(cloud)
+        dev = cloud._device
+        L = _lib.lib()
+        counts = torch.zeros(self.ngrid, dtype=torch.int64, device=dev)
+        _lib.check(L.vt_project_count(cloud._handle(), _P(s.data_ptr()),
+                                      _P(e.data_ptr()), self.ngrid,
+                                      _P(counts.data_ptr()), _stream()))
+        offsets = torch.zeros(self.ngrid + 1, dtype=torch.int64, device=dev)
+        torch.cumsum(counts, 0, out=offsets[1:])
+        total = int(offsets[-1].item())
+        vals = torch.zeros((self.ngrid, self._nout(cloud, reduction)),
+                           dtype=torch.float64, device=dev)
+        ids = torch.empty(max(total, 1), dtype=torch.int64, device=dev)
+        se = torch.empty(max(total, 1), dtype=torch.float64, device=dev)
+        sx = torch.empty(max(total, 1), dtype=torch.float64, device=dev)
+        _lib.check(L.vt_project_segments(
+            cloud._handle(), _P(s.data_ptr()), _P(e.data_ptr()), self.ngrid,
+            int(reduction), _P(offsets.data_ptr()), _P(vals.data_ptr()),
+            _P(ids.data_ptr()), _P(se.data_ptr()), _P(sx.data_ptr()), _stream()))
+        return vals, ids[:total], se[:total], sx[:total], offsets
+
+    def makeDetailedProjection(self, cloud: PointCloud, reduction=Sum):
+        v, ids, se, sx, off = self.segments_tensor(cloud, reduction)
+        self._data = v
+        self._segments = (ids, se, sx, off)
+
+    def returnSegments(self):
+        if self._segments is None:
+            raise RuntimeError("No detailed projection data available")
+        return tuple(t.cpu().numpy() for t in self._segments)

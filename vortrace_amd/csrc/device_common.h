@@ -1,0 +1,124 @@
+// Device-side definitions shared by the kNN and walk code (gfx950, FP64).
+//
+// All arithmetic that decides a cell id or an s value is FP64 with every
+// multiply and add rounded separately (the translation unit is compiled with
+// -ffp-contract=off and `#pragma clang fp contract(off)`), in the operand
+// order of the reference, so results are bit-identical to a default x86-64
+// build of it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "layout.h"
+
+#pragma clang fp contract(off)
+
+namespace vt {
+
+// src/ray.cpp:8-12 (TOLERANCE, PARALLEL_BISECTOR_TOL, MAX_ITERATIONS,
+// PERTURBATION_EPS = 2 * TOLERANCE, MAX_PERTURBATION_CYCLES).
+constexpr double kTol = 1E-9;
+constexpr double kParTol = 1E-12;
+constexpr int kMaxIter = 100000;
+constexpr double kPertEps = 2 * kTol;
+constexpr int kMaxCycles = 1000;
+constexpr int kExpandK = 100;  // K_EXPAND, src/ray.cpp:321
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kUndo = 0x80000000u;
+
+// Error codes raised inside kernels; the host maps them to the reference's
+// exception texts.
+enum : int {
+  kErrNone = 0,
+  kErrMaxIter = 1,         // src/ray.cpp:260-261
+  kErrSplitOutside = 2,    // src/ray.cpp:282-289
+  kErrClassify = 3,        // src/ray.cpp:331-336
+  kErrBisect = 4,          // src/ray.cpp:125-126
+  kErrPerturb = 5,         // src/ray.cpp:182-184
+  kErrPeriodicStart = 6,   // src/ray.cpp:243-246
+  kErrWalkStack = 7,       // walk stack deeper than the allocated capacity
+  kErrKnnStack = 8,        // kNN stack deeper than the tree (cannot happen)
+  kErrTies = 9,            // >= K_EXPAND equidistant generators at one point
+  kErrNonFinite = 10,      // a query found no generator (non-finite point)
+  kErrFillOverflow = 11,   // fill pass produced more segments than counted
+};
+
+struct ErrRec {
+  int flag;  // 0 until the first error claims the record
+  int code;
+  int64_t ray;
+  double v[3];
+  uint32_t id[2];
+  int pad[2];
+};
+
+struct Stats {
+  unsigned long long segments;
+  unsigned long long queries;
+  unsigned int max_walk_depth;
+  unsigned int max_knn_depth;
+  unsigned long long slow_queries;
+  unsigned long long ray_counter;  // work queue head
+};
+
+// Everything a kernel needs to know about a built cloud (passed by value).
+struct DevCloud {
+  const Node *nodes;
+  const Gen *gen;        // slot order
+  const double *fields;  // slot order, nf per slot
+  double rb_lo[3], rb_hi[3];  // nanoflann root_bbox
+  double sb[6];               // PointCloud::subbox
+  double box[3];              // periodic box size
+  double inv_sqrt2;           // 1.0 / std::sqrt(2.0), computed on the host
+  uint32_t n, nf;
+  int periodic;
+  int pad;
+};
+
+// Per-lane scratch that does not fit in registers/LDS (device memory).
+struct Scratch {
+  uint32_t *knn_tag;   // [kspill][T] kNN stack entries beyond the LDS part
+  double *knn_val;
+  uint32_t *walk_tag;  // [wcap][T] pending ray points below the top
+  double *walk_s;
+  uint32_t *big_id;    // [T][3 * kMaxQueryK] k > 3 result / merge buffers
+  double *big_d;
+  int kspill, wcap;
+  int lanes;           // T
+  int pad;
+};
+
+constexpr int kMaxQueryK = 128;
+
+__device__ __forceinline__ Node load_node(const Node *nodes, uint32_t i) {
+  const double2 *p = reinterpret_cast<const double2 *>(nodes + i);
+  const double2 v0 = p[0];
+  const double2 v1 = p[1];
+  Node n;
+  n.lo = v0.x;
+  n.hi = v0.y;
+  uint64_t ab = __double_as_longlong(v1.x);
+  n.a = static_cast<uint32_t>(ab);
+  n.b = static_cast<uint32_t>(ab >> 32);
+  n.depth = 0;
+  n.pad = 0;
+  return n;
+}
+
+__device__ __forceinline__ void report_error(ErrRec *err, int code, int64_t ray, double a,
+                                             double b, double c, uint32_t i0, uint32_t i1) {
+  if (atomicCAS(&err->flag, 0, 1) == 0) {
+    err->code = code;
+    err->ray = ray;
+    err->v[0] = a;
+    err->v[1] = b;
+    err->v[2] = c;
+    err->id[0] = i0;
+    err->id[1] = i1;
+    __threadfence();
+  }
+}
+
+}  // namespace vt

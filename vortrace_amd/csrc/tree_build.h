@@ -1,0 +1,25 @@
+// Host builder of the device kd-tree (see layout.h).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "layout.h"
+
+namespace vt {
+
+struct HostTree {
+  std::vector<Node> nodes;       // pre-order
+  std::vector<uint32_t> vind;    // slot -> filtered index
+  double bbox[3][2];             // root bounding box (nanoflann root_bbox)
+  int64_t leaves = 0;
+  int64_t max_depth = 0;
+};
+
+// Builds the tree over pts (n x 3, filtered order).  Every decision (cut
+// dimension, cut value, partition and split index) follows nanoflann 1.3.2's
+// buildIndex, so the slot order and every divlow/divhigh equal the
+// reference's.  Uses up to `threads` host threads for independent subtrees.
+void build_tree(const double *pts, size_t n, int threads, HostTree &out);
+
+}  // namespace vt

@@ -1,0 +1,1017 @@
+// libvortrace_hip: HIP kernels for gfx950 and the C-ABI of include/vortrace_hip.h.
+//
+// Kernels (one ray or one query per lane, persistent grid, rays pulled from a
+// device work counter):
+//   k_query      PointCloud::queryTreeK           (src/pointcloud.cpp:308-368)
+//   k_project    Projection::makeProjection       (src/projection.cpp:26-73)
+//                fused with reduce_sum / reduce_max / reduce_min /
+//                reduce_volume_render             (src/reduction.cpp:8-83)
+//   k_count      segment counts of makeDetailedProjection
+//   k_fill       segments + values of makeDetailedProjection
+//                                                 (src/projection.cpp:75-144)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/vortrace_hip.h"
+#include "tree_build.h"
+#include "walk.h"
+
+#pragma clang fp contract(off)
+
+namespace vt {
+
+constexpr int kBlock = 128;    // lanes per workgroup (2 waves)
+constexpr int kLdsDepth = 16;  // kNN stack entries per lane held in LDS
+constexpr int kWalkCap = 128;  // pending ray points per lane (device memory)
+
+using Stack = KStack<kLdsDepth, kBlock>;
+using Ctx = WalkCtx<kLdsDepth, kBlock>;
+
+// ---------------------------------------------------------------- sinks ----
+
+template <int NF>
+struct SumSink {  // reduce_sum, src/reduction.cpp:8-21
+  const double *fields;
+  double acc[NF];
+  uint64_t n;
+  __device__ void init() {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = 0.0;
+    n = 0;
+  }
+  __device__ __forceinline__ void seg(uint32_t cell, double a, double b) {
+    const double ds = b - a;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) acc[f] = acc[f] + ds * fields[(size_t)cell * NF + f];
+    ++n;
+  }
+  __device__ void store(double *out) const {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) out[f] = acc[f];
+  }
+};
+
+struct SumSinkDyn {  // reduce_sum for any number of fields, accumulated in place
+  const double *fields;
+  int nf;
+  double *out;
+  uint64_t n;
+  __device__ void init() {
+    for (int f = 0; f < nf; ++f) out[f] = 0.0;
+    n = 0;
+  }
+  __device__ void seg(uint32_t cell, double a, double b) {
+    const double ds = b - a;
+    for (int f = 0; f < nf; ++f) out[f] = out[f] + ds * fields[(size_t)cell * nf + f];
+    ++n;
+  }
+  __device__ void store(double *) const {}
+};
+
+template <bool kMax>
+struct ExtSink {  // reduce_max / reduce_min, src/reduction.cpp:23-53
+  const double *fields;
+  int nf;
+  double *out;
+  uint64_t n;
+  __device__ void init() {
+    for (int f = 0; f < nf; ++f) out[f] = kMax ? -INFINITY : INFINITY;
+    n = 0;
+  }
+  __device__ void seg(uint32_t cell, double, double) {
+    for (int f = 0; f < nf; ++f) {
+      const double v = fields[(size_t)cell * nf + f];
+      if (kMax ? (v > out[f]) : (v < out[f])) out[f] = v;
+    }
+    ++n;
+  }
+  __device__ void store(double *) const {}
+};
+
+struct VolSink {  // reduce_volume_render, src/reduction.cpp:55-83
+  const double *fields;
+  double T, c[3];
+  uint64_t n;
+  __device__ void init() {
+    T = 1.0;
+    c[0] = c[1] = c[2] = 0.0;
+    n = 0;
+  }
+  __device__ __forceinline__ void seg(uint32_t cell, double a, double b) {
+    const double *fv = fields + (size_t)cell * 4;
+    const double alpha = fv[3];
+    const double ds = b - a;
+    const double opacity = 1.0 - exp(-alpha * ds);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] += T * fv[k] * opacity;
+    T *= (1.0 - opacity);
+    ++n;
+  }
+  __device__ void store(double *out) const {
+    out[0] = c[0];
+    out[1] = c[1];
+    out[2] = c[2];
+  }
+};
+
+struct CountSink {
+  uint64_t n;
+  __device__ void init() { n = 0; }
+  __device__ __forceinline__ void seg(uint32_t, double, double) { ++n; }
+  __device__ void store(double *) const {}
+};
+
+template <class Inner>
+struct FillSink {  // segments at the ray's offset, plus the reduced values
+  Inner in;
+  const uint32_t *slot_id;
+  int64_t *ids;
+  double *se, *sx;
+  uint64_t cap;  // room the count pass reserved for this ray
+  uint64_t n;
+  __device__ void init() {
+    in.init();
+    n = 0;
+  }
+  __device__ __forceinline__ void seg(uint32_t cell, double a, double b) {
+    if (n < cap) {
+      ids[n] = (int64_t)slot_id[cell];
+      se[n] = a;
+      sx[n] = b;
+    }
+    ++n;
+    in.seg(cell, a, b);
+  }
+  __device__ void store(double *out) const { in.store(out); }
+};
+
+// -------------------------------------------------------------- kernels ----
+
+struct LaneState {
+  __device__ static int gtid() { return blockIdx.x * kBlock + threadIdx.x; }
+};
+
+__device__ __forceinline__ void flush_stats(Stats *stats, uint64_t nseg, uint64_t nq,
+                                            uint64_t nslow, int maxw, int maxk) {
+  atomicAdd(&stats->segments, (unsigned long long)nseg);
+  atomicAdd(&stats->queries, (unsigned long long)nq);
+  if (nslow) atomicAdd(&stats->slow_queries, (unsigned long long)nslow);
+  atomicMax(&stats->max_walk_depth, (unsigned int)maxw);
+  atomicMax(&stats->max_knn_depth, (unsigned int)maxk);
+}
+
+#define VT_LANE_SETUP                                                                      \
+  __shared__ uint32_t lds_tag[kLdsDepth * kBlock];                                         \
+  __shared__ double lds_val[kLdsDepth * kBlock];                                           \
+  const int tid = threadIdx.x;                                                             \
+  const int gtid = blockIdx.x * kBlock + tid;                                              \
+  Ctx x{C, Stack{lds_tag + tid, lds_val + tid, S.knn_tag + gtid, S.knn_val + gtid, S.lanes, \
+                 S.kspill, 0, 0, false},                                                   \
+        S, gtid, (int)(tid & 63), gtid >> 6, 0, 0, false};
+
+template <int MODE, int NF>
+__global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const double *start,
+                                                    const double *end, int64_t nrays,
+                                                    double *out, int nout, ErrRec *err,
+                                                    Stats *stats) {
+  VT_LANE_SETUP
+  uint64_t nseg = 0;
+  int maxw = 0;
+  for (;;) {
+    const int64_t ray = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
+    if (ray >= nrays) break;
+    if (__hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+    RayGeom r;
+    make_ray(start + 3 * ray, end + 3 * ray, r);
+    double ev[3] = {0, 0, 0};
+    uint32_t ei[2] = {0, 0};
+    int rc;
+    double *o = out + ray * nout;
+    if constexpr (MODE == VT_SUM && NF > 0) {
+      SumSink<NF> sk{C.fields};
+      sk.init();
+      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      sk.store(o);
+      nseg += sk.n;
+    } else if constexpr (MODE == VT_SUM) {
+      SumSinkDyn sk{C.fields, (int)C.nf, o};
+      sk.init();
+      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      nseg += sk.n;
+    } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
+      ExtSink<MODE == VT_MAX> sk{C.fields, (int)C.nf, o};
+      sk.init();
+      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      nseg += sk.n;
+    } else {
+      VolSink sk{C.fields};
+      sk.init();
+      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      sk.store(o);
+      nseg += sk.n;
+    }
+    if (!rc && x.st.overflow) rc = kErrKnnStack;
+    if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
+  }
+  flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
+}
+
+__global__ __launch_bounds__(kBlock) void k_count(DevCloud C, Scratch S, const double *start,
+                                                  const double *end, int64_t nrays,
+                                                  int64_t *counts, ErrRec *err, Stats *stats) {
+  VT_LANE_SETUP
+  uint64_t nseg = 0;
+  int maxw = 0;
+  for (;;) {
+    const int64_t ray = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
+    if (ray >= nrays) break;
+    if (__hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+    RayGeom r;
+    make_ray(start + 3 * ray, end + 3 * ray, r);
+    double ev[3] = {0, 0, 0};
+    uint32_t ei[2] = {0, 0};
+    CountSink sk;
+    sk.init();
+    int rc = walk_ray(x, r, sk, maxw, ev, ei);
+    if (!rc && x.st.overflow) rc = kErrKnnStack;
+    counts[ray] = (int64_t)sk.n;
+    nseg += sk.n;
+    if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
+  }
+  flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_fill(DevCloud C, Scratch S, const double *start,
+                                                 const double *end, int64_t nrays,
+                                                 const int64_t *offsets, double *out, int nout,
+                                                 const uint32_t *slot_id, int64_t *ids,
+                                                 double *se, double *sx, ErrRec *err,
+                                                 Stats *stats) {
+  VT_LANE_SETUP
+  uint64_t nseg = 0;
+  int maxw = 0;
+  for (;;) {
+    const int64_t ray = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
+    if (ray >= nrays) break;
+    if (__hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
+    RayGeom r;
+    make_ray(start + 3 * ray, end + 3 * ray, r);
+    double ev[3] = {0, 0, 0};
+    uint32_t ei[2] = {0, 0};
+    const int64_t off = offsets[ray];
+    const int64_t cap = offsets[ray + 1] - off;
+    double *o = out + ray * nout;
+    int rc;
+    uint64_t n;
+    if constexpr (MODE == VT_SUM) {
+      FillSink<SumSinkDyn> sk{SumSinkDyn{C.fields, (int)C.nf, o}, slot_id, ids + off, se + off,
+                              sx + off, (uint64_t)cap};
+      sk.init();
+      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      n = sk.n;
+    } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
+      FillSink<ExtSink<MODE == VT_MAX>> sk{ExtSink<MODE == VT_MAX>{C.fields, (int)C.nf, o},
+                                           slot_id, ids + off, se + off, sx + off,
+                                           (uint64_t)cap};
+      sk.init();
+      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      n = sk.n;
+    } else {
+      FillSink<VolSink> sk{VolSink{C.fields}, slot_id, ids + off, se + off, sx + off,
+                           (uint64_t)cap};
+      sk.init();
+      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      sk.store(o);
+      n = sk.n;
+    }
+    nseg += n;
+    if (!rc && x.st.overflow) rc = kErrKnnStack;
+    if (!rc && (int64_t)n != cap) rc = kErrFillOverflow;
+    if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
+  }
+  flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_query_reg(DevCloud C, Scratch S, const double *q,
+                                                      int64_t nq, const uint32_t *slot_id,
+                                                      int64_t *ids, double *r2, ErrRec *err,
+                                                      Stats *stats) {
+  VT_LANE_SETUP
+  for (;;) {
+    const int64_t i = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
+    if (i >= nq) break;
+    uint32_t id[K];
+    double d[K];
+    query_reg<K>(C, q[3 * i], q[3 * i + 1], q[3 * i + 2], id, d, x.st);
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      ids[i * K + j] = id[j] == kNone ? -1 : (int64_t)slot_id[id[j]];
+      r2[i * K + j] = d[j];
+    }
+  }
+  if (x.st.overflow) report_error(err, kErrKnnStack, 0, 0, 0, 0, 0, 0);
+  flush_stats(stats, 0, 0, 0, 0, x.st.maxsp);
+}
+
+__global__ __launch_bounds__(kBlock) void k_query_mem(DevCloud C, Scratch S, const double *q,
+                                                      int64_t nq, int k, uint32_t *wid,
+                                                      double *wd, const uint32_t *slot_id,
+                                                      int64_t *ids, double *r2, ErrRec *err,
+                                                      Stats *stats) {
+  VT_LANE_SETUP
+  uint32_t *bi = wid + (size_t)gtid * 4 * k;
+  double *bd = wd + (size_t)gtid * 4 * k;
+  for (;;) {
+    const int64_t i = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
+    if (i >= nq) break;
+    MemRes res{bd, bi, k, 0};
+    query_mem(C, q[3 * i], q[3 * i + 1], q[3 * i + 2], k, res, bi + 2 * k, bd + 2 * k, x.st);
+    for (int j = 0; j < k; ++j) {
+      ids[i * k + j] = bi[j] == kNone ? -1 : (int64_t)slot_id[bi[j]];
+      r2[i * k + j] = bd[j];
+    }
+  }
+  if (x.st.overflow) report_error(err, kErrKnnStack, 0, 0, 0, 0, 0, 0);
+  flush_stats(stats, 0, 0, 0, 0, x.st.maxsp);
+}
+
+}  // namespace vt
+
+// ================================================================ host ====
+
+using namespace vt;
+
+namespace {
+
+thread_local std::string g_err;
+thread_local vt_call_stats g_stats{};
+std::mutex g_warn_mu;
+vt_warning_fn g_warn_fn = nullptr;
+void *g_warn_user = nullptr;
+std::atomic<int> g_verbose{0};
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+void warn(const std::string &msg) {
+  std::lock_guard<std::mutex> lk(g_warn_mu);
+  if (g_warn_fn)
+    g_warn_fn(msg.c_str(), g_warn_user);
+  else
+    std::fprintf(stderr, "vortrace warning: %s\n", msg.c_str());
+}
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIP_OK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      throw HipError(std::string("HIP error in " #expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+T *dev_alloc(size_t count) {
+  void *p = nullptr;
+  if (count == 0) count = 1;
+  HIP_OK(hipMalloc(&p, count * sizeof(T)));
+  return static_cast<T *>(p);
+}
+
+std::string fmt(const char *f, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, f);
+  std::vsnprintf(buf, sizeof buf, f, ap);
+  va_end(ap);
+  return buf;
+}
+
+}  // namespace
+
+struct vt_cloud {
+  // host copy (filtered order), PointCloud members of include/pointcloud.hpp:21-32
+  size_t n = 0, nf = 1;
+  std::vector<double> pts, fields;
+  std::vector<int64_t> orig_ids;
+  double subbox[6] = {0, 0, 0, 0, 0, 0};
+  double box[3] = {0, 0, 0};
+  double pad = 0.0;
+  int periodic = 0, valid_rgba = 0;
+  bool tree_built = false;  // host tree built and uploaded
+  bool host_tree = false;   // host tree built
+  HostTree tree;
+  double build_seconds = 0;
+
+  // device copy
+  int device = -1;
+  Node *d_nodes = nullptr;
+  Gen *d_gen = nullptr;
+  double *d_fields = nullptr;
+  uint32_t *d_slot_id = nullptr;
+  ErrRec *d_err = nullptr;
+  Stats *d_stats = nullptr;
+  Scratch scratch{};
+  size_t scratch_lanes = 0;
+  int kspill = 0;
+  DevCloud dc{};
+
+  void free_device() {
+    for (void *p : {(void *)d_nodes, (void *)d_gen, (void *)d_fields, (void *)d_slot_id,
+                    (void *)d_err, (void *)d_stats, (void *)scratch.knn_tag,
+                    (void *)scratch.knn_val, (void *)scratch.walk_tag, (void *)scratch.walk_s,
+                    (void *)scratch.big_id, (void *)scratch.big_d})
+      if (p) (void)hipFree(p);
+    d_nodes = nullptr;
+    d_gen = nullptr;
+    d_fields = nullptr;
+    d_slot_id = nullptr;
+    d_err = nullptr;
+    d_stats = nullptr;
+    scratch = Scratch{};
+    scratch_lanes = 0;
+  }
+  ~vt_cloud() { free_device(); }
+};
+
+namespace {
+
+int num_cus(int dev) {
+  hipDeviceProp_t p;
+  HIP_OK(hipGetDeviceProperties(&p, dev));
+  return p.multiProcessorCount;
+}
+
+// Persistent grid size for a kernel: as many workgroups as fit on the device
+// at once, but no more than the work needs.
+template <class K>
+int grid_for(K kernel, int64_t items, int dev) {
+  int per_cu = 0;
+  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+  if (per_cu < 1) per_cu = 1;
+  int64_t full = (int64_t)per_cu * num_cus(dev);
+  int64_t need = (items + kBlock - 1) / kBlock;
+  return (int)std::max<int64_t>(1, std::min(full, need));
+}
+
+// Per-lane device scratch for `lanes` persistent lanes.
+void ensure_scratch(vt_cloud *c, size_t lanes) {
+  if (lanes <= c->scratch_lanes) return;
+  for (void *p : {(void *)c->scratch.knn_tag, (void *)c->scratch.knn_val,
+                  (void *)c->scratch.walk_tag, (void *)c->scratch.walk_s,
+                  (void *)c->scratch.big_id, (void *)c->scratch.big_d})
+    if (p) (void)hipFree(p);
+  Scratch s{};
+  s.kspill = c->kspill;
+  s.wcap = kWalkCap;
+  s.lanes = (int)lanes;
+  s.knn_tag = dev_alloc<uint32_t>((size_t)s.kspill * lanes);
+  s.knn_val = dev_alloc<double>((size_t)s.kspill * lanes);
+  s.walk_tag = dev_alloc<uint32_t>((size_t)s.wcap * lanes);
+  s.walk_s = dev_alloc<double>((size_t)s.wcap * lanes);
+  const size_t waves = (lanes + 63) / 64;
+  s.big_id = dev_alloc<uint32_t>(waves * 4 * kMaxQueryK);
+  s.big_d = dev_alloc<double>(waves * 4 * kMaxQueryK);
+  c->scratch = s;
+  c->scratch_lanes = lanes;
+}
+
+void reset_call(vt_cloud *c, hipStream_t st) {
+  HIP_OK(hipMemsetAsync(c->d_err, 0, sizeof(ErrRec), st));
+  HIP_OK(hipMemsetAsync(c->d_stats, 0, sizeof(Stats), st));
+}
+
+// Waits for the call, copies the error record and statistics back and turns
+// a kernel error into the reference's message.
+int finish_call(vt_cloud *c, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+  ErrRec er;
+  Stats ss;
+  HIP_OK(hipMemcpyAsync(&er, c->d_err, sizeof er, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(&ss, c->d_stats, sizeof ss, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  float ms = 0;
+  if (e0 && e1) HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  g_stats.segments = (int64_t)ss.segments;
+  g_stats.queries = (int64_t)ss.queries;
+  g_stats.max_walk_depth = ss.max_walk_depth;
+  g_stats.max_knn_depth = ss.max_knn_depth;
+  g_stats.slow_queries = (int64_t)ss.slow_queries;
+  g_stats.kernel_ms = ms;
+  if (!er.flag) return VT_OK;
+  auto cell = [&](uint32_t slot) -> size_t {
+    return slot < c->tree.vind.size() ? (size_t)c->tree.vind[slot] : (size_t)slot;
+  };
+  switch (er.code) {
+    case kErrMaxIter:
+      return fail(VT_ERUNTIME, "Ray walk exceeded maximum iteration count");
+    case kErrSplitOutside:
+      return fail(VT_ERUNTIME, fmt("Split point outside segment bounds: s=%f not in (%f, %f) "
+                                   "for cells %zu and %zu",
+                                   er.v[0], er.v[1], er.v[2], cell(er.id[0]), cell(er.id[1])));
+    case kErrClassify:
+      return fail(VT_ERUNTIME,
+                  "Could not resolve split point classification after expanding to 100-NN. "
+                  "This should be very unlikely; consider increasing TOLERANCE or K_EXPAND.");
+    case kErrBisect:
+      return fail(VT_ERUNTIME, fmt("Binary search did not converge for split point between "
+                                   "cells %zu and %zu",
+                                   cell(er.id[0]), cell(er.id[1])));
+    case kErrPerturb:
+      return fail(VT_ERUNTIME,
+                  "Perturbation did not resolve degenerate split point after 1000 cycles");
+    case kErrPeriodicStart:
+      return fail(VT_ERUNTIME, "Could not find intermediate cell by bisection along periodic ray");
+    case kErrWalkStack:
+      return fail(VT_ERUNTIME, fmt("Ray %lld: more than %d pending split points on one ray "
+                                   "(walk stack capacity)",
+                                   (long long)er.ray, kWalkCap));
+    case kErrNonFinite:
+      return fail(VT_EINVAL, fmt("Ray %lld: a query point has non-finite coordinates",
+                                 (long long)er.ray));
+    case kErrFillOverflow:
+      return fail(VT_ERUNTIME, fmt("Ray %lld: segment count changed between passes",
+                                   (long long)er.ray));
+    case kErrTies:
+      return fail(VT_ERUNTIME, fmt("Ray %lld: 100 or more generators equidistant from one "
+                                   "split point; classification is ill-defined",
+                                   (long long)er.ray));
+    default:
+      return fail(VT_ERUNTIME, fmt("Ray %lld: kNN traversal stack overflow", (long long)er.ray));
+  }
+}
+
+struct Timer {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  Timer() {
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+  }
+  ~Timer() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
+int check_ready(const vt_cloud *c) {
+  if (!c) return fail(VT_EINVAL, "null cloud");
+  if (!c->tree_built)
+    return fail(VT_ERUNTIME, "There is currently no valid tree for this point cloud");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != c->device)
+    return fail(VT_ERUNTIME, "the cloud's tree lives on another HIP device than the current one");
+  return VT_OK;
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    return f();
+  } catch (const std::invalid_argument &e) {
+    return fail(VT_EINVAL, e.what());
+  } catch (const std::exception &e) {
+    return fail(VT_ERUNTIME, e.what());
+  }
+}
+
+}  // namespace
+
+namespace {
+
+template <int MODE, int NF>
+void launch_project(vt_cloud *c, const double *s, const double *e, int64_t n, double *out,
+                    int nout, hipStream_t st, Timer &tm) {
+  const int grid = grid_for(k_project<MODE, NF>, n, c->device);
+  ensure_scratch(c, (size_t)grid * kBlock);
+  HIP_OK(hipEventRecord(tm.e0, st));
+  k_project<MODE, NF><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, s, e, n, out, nout, c->d_err,
+                                               c->d_stats);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(tm.e1, st));
+}
+
+int check_mode(const vt_cloud *c, int mode) {
+  if (mode < VT_SUM || mode > VT_VOLUME) return fail(VT_EINVAL, "Unrecognized reduction mode");
+  if (mode == VT_VOLUME) {
+    if (c->nf != 4)
+      return fail(VT_EINVAL, fmt("VolumeRender requires exactly 4 fields (R, G, B, alpha), "
+                                 "got %zu",
+                                 c->nf));
+    if (!c->valid_rgba)
+      return fail(VT_EINVAL, "Volume rendering requires R, G, B values in [0, 1] and alpha >= 0");
+  }
+  return VT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vt_last_error(void) { return g_err.c_str(); }
+
+void vt_set_warning_callback(vt_warning_fn fn, void *user) {
+  std::lock_guard<std::mutex> lk(g_warn_mu);
+  g_warn_fn = fn;
+  g_warn_user = user;
+}
+
+void vt_set_verbose(int v) { g_verbose.store(v ? 1 : 0); }
+int vt_get_verbose(void) { return g_verbose.load(); }
+
+int vt_cloud_create(const double *pos, size_t npart, const double *fields, size_t nfields,
+                    const double subbox[6], const double *vol, size_t nvol, int periodic,
+                    vt_cloud **out) {
+  return guarded([&]() -> int {
+    if (!out || !subbox) return fail(VT_EINVAL, "null argument");
+    if (npart && (!pos || !fields)) return fail(VT_EINVAL, "null argument");
+    auto c = std::make_unique<vt_cloud>();
+    c->nf = nfields;
+    std::memcpy(c->subbox, subbox, sizeof c->subbox);
+    c->periodic = periodic ? 1 : 0;
+    if (c->periodic) {
+      c->box[0] = subbox[1] - subbox[0];
+      c->box[1] = subbox[3] - subbox[2];
+      c->box[2] = subbox[5] - subbox[4];
+    }
+    // src/pointcloud.cpp:38-53
+    if (nvol > 0) {
+      double min_vol = std::numeric_limits<double>::max();
+      for (size_t i = 0; i < nvol; i++)
+        if (vol[i] > 0 && vol[i] < min_vol) min_vol = vol[i];
+      if (min_vol < std::numeric_limits<double>::max()) {
+        double min_radius = std::cbrt(3.0 * min_vol / (4.0 * M_PI));
+        if (min_radius < 1e-6)
+          warn("Some cells have radii smaller than 1e-6, which approaches the internal "
+               "ray-tracing tolerance (1e-9). Results may be unreliable for very small cells. "
+               "Consider rescaling your coordinate system.");
+      }
+    }
+    if (!c->periodic) {
+      // Padded filter, src/pointcloud.cpp:55-108.
+      const double dx = subbox[1] - subbox[0], dy = subbox[3] - subbox[2],
+                   dz = subbox[5] - subbox[4];
+      if (nvol > 0) {
+        double max_vol = 0.0;
+        for (size_t i = 0; i < nvol; i++)
+          if (vol[i] > max_vol) max_vol = vol[i];
+        c->pad = 10.0 * std::cbrt(3.0 * max_vol / (4.0 * M_PI));
+      } else {
+        c->pad = 0.15 * std::max({dx, dy, dz});
+        warn("No cell volumes provided; using 15% of longest box side as padding. Pass vol= "
+             "for adaptive padding.");
+      }
+      const double lo[3] = {subbox[0] - c->pad, subbox[2] - c->pad, subbox[4] - c->pad};
+      const double hi[3] = {subbox[1] + c->pad, subbox[3] + c->pad, subbox[5] + c->pad};
+      for (size_t i = 0; i < npart; i++) {
+        const double *p = pos + 3 * i;
+        if (p[0] >= lo[0] && p[0] <= hi[0] && p[1] >= lo[1] && p[1] <= hi[1] && p[2] >= lo[2] &&
+            p[2] <= hi[2]) {
+          c->pts.insert(c->pts.end(), p, p + 3);
+          c->fields.insert(c->fields.end(), fields + i * nfields, fields + (i + 1) * nfields);
+          c->orig_ids.push_back((int64_t)i);
+        }
+      }
+    } else {
+      // All points, validated against the periodic domain, src/pointcloud.cpp:109-163.
+      double pmin[3], pmax[3];
+      for (int d = 0; d < 3; d++) {
+        pmin[d] = std::numeric_limits<double>::max();
+        pmax[d] = std::numeric_limits<double>::lowest();
+      }
+      c->pts.assign(pos, pos + 3 * npart);
+      c->fields.assign(fields, fields + npart * nfields);
+      c->orig_ids.resize(npart);
+      for (size_t i = 0; i < npart; i++) {
+        c->orig_ids[i] = (int64_t)i;
+        for (int d = 0; d < 3; d++) {
+          pmin[d] = std::min(pmin[d], pos[3 * i + d]);
+          pmax[d] = std::max(pmax[d], pos[3 * i + d]);
+        }
+      }
+      if (pmin[0] < subbox[0] || pmax[0] > subbox[1] || pmin[1] < subbox[2] ||
+          pmax[1] > subbox[3] || pmin[2] < subbox[4] || pmax[2] > subbox[5])
+        return fail(VT_EINVAL,
+                    "Periodic mode: some particles lie outside the bounding box. All particles "
+                    "must be within the periodic domain.");
+      for (int d = 0; d < 3; d++)
+        if (pmax[d] - pmin[d] < 0.6 * c->box[d]) {
+          warn("Periodic mode: particle extent is less than 60% of the bounding box in at "
+               "least one dimension. The bounding box may be too large for the particle "
+               "distribution.");
+          break;
+        }
+    }
+    c->n = c->orig_ids.size();
+    // src/pointcloud.cpp:167-179
+    c->valid_rgba = 0;
+    if (c->nf == 4 && c->n > 0) {
+      c->valid_rgba = 1;
+      for (size_t i = 0; i < c->n && c->valid_rgba; i++) {
+        for (int k = 0; k < 3; k++) {
+          const double v = c->fields[i * 4 + k];
+          if (v < 0.0 || v > 1.0) {
+            c->valid_rgba = 0;
+            break;
+          }
+        }
+        if (c->valid_rgba && c->fields[i * 4 + 3] < 0.0) c->valid_rgba = 0;
+      }
+    }
+    *out = c.release();
+    return VT_OK;
+  });
+}
+
+int vt_cloud_build_host_tree(vt_cloud *c) {
+  return guarded([&]() -> int {
+    if (!c) return fail(VT_EINVAL, "null cloud");
+    if (c->n == 0) return fail(VT_ERUNTIME, "There are no points in the cloud; cannot build tree");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    HostTree t;
+    build_tree(c->pts.data(), c->n, threads, t);
+    c->build_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    c->free_device();
+    c->tree_built = false;
+    c->host_tree = true;
+    c->tree = std::move(t);
+    if (g_verbose.load())
+      std::printf("Tree build took %lld milliseconds.\n", (long long)(c->build_seconds * 1e3));
+    return VT_OK;
+  });
+}
+
+int vt_cloud_upload(vt_cloud *c) {
+  return guarded([&]() -> int {
+    if (!c) return fail(VT_EINVAL, "null cloud");
+    if (!c->host_tree)
+      return fail(VT_ERUNTIME, "There is currently no valid tree for this point cloud");
+    const HostTree &t = c->tree;
+    // Slot-ordered generators and fields.
+    std::vector<Gen> gen(c->n);
+    std::vector<double> fslot(c->n * c->nf);
+    for (size_t s = 0; s < c->n; ++s) {
+      const size_t i = t.vind[s];
+      gen[s] = Gen{c->pts[3 * i], c->pts[3 * i + 1], c->pts[3 * i + 2], 0.0};
+      std::memcpy(&fslot[s * c->nf], &c->fields[i * c->nf], c->nf * sizeof(double));
+    }
+    c->free_device();
+    c->tree_built = false;
+    HIP_OK(hipGetDevice(&c->device));
+    c->d_nodes = dev_alloc<Node>(t.nodes.size());
+    c->d_gen = dev_alloc<Gen>(c->n);
+    c->d_fields = dev_alloc<double>(fslot.size());
+    c->d_slot_id = dev_alloc<uint32_t>(c->n);
+    c->d_err = dev_alloc<ErrRec>(1);
+    c->d_stats = dev_alloc<Stats>(1);
+    HIP_OK(hipMemcpy(c->d_nodes, t.nodes.data(), t.nodes.size() * sizeof(Node),
+                     hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_gen, gen.data(), gen.size() * sizeof(Gen), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_fields, fslot.data(), std::max<size_t>(1, fslot.size()) * sizeof(double),
+                     hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->d_slot_id, t.vind.data(), c->n * sizeof(uint32_t),
+                     hipMemcpyHostToDevice));
+    // kNN stack: at most one entry per level of the current path.
+    c->kspill = std::max<int>(1, (int)t.max_depth + 2 - kLdsDepth);
+    DevCloud &d = c->dc;
+    d.nodes = c->d_nodes;
+    d.gen = c->d_gen;
+    d.fields = c->d_fields;
+    for (int k = 0; k < 3; ++k) {
+      d.rb_lo[k] = t.bbox[k][0];
+      d.rb_hi[k] = t.bbox[k][1];
+      d.box[k] = c->box[k];
+    }
+    std::memcpy(d.sb, c->subbox, sizeof d.sb);
+    d.inv_sqrt2 = 1.0 / std::sqrt(2.0);
+    d.n = (uint32_t)c->n;
+    d.nf = (uint32_t)c->nf;
+    d.periodic = c->periodic;
+    c->tree_built = true;
+    return VT_OK;
+  });
+}
+
+int vt_cloud_build_tree(vt_cloud *c) {
+  int rc = vt_cloud_build_host_tree(c);
+  return rc ? rc : vt_cloud_upload(c);
+}
+
+int vt_cloud_tree_export(const vt_cloud *c, int64_t *vind, int32_t *feat, int64_t *a,
+                         int64_t *b, double *divlow, double *divhigh, double *bbox6) {
+  if (!c) return fail(VT_EINVAL, "null cloud");
+  if (!c->host_tree)
+    return fail(VT_ERUNTIME, "There is currently no valid tree for this point cloud");
+  const HostTree &t = c->tree;
+  for (size_t i = 0; i < c->n; ++i) vind[i] = (int64_t)t.vind[i];
+  for (size_t i = 0; i < t.nodes.size(); ++i) {
+    const Node &nd = t.nodes[i];
+    if (nd.b & kLeafBit) {
+      feat[i] = -1;
+      a[i] = nd.a;
+      b[i] = nd.b & ~kLeafBit;
+      divlow[i] = divhigh[i] = 0.0;
+    } else {
+      feat[i] = (int32_t)nd.b;
+      a[i] = (int64_t)i + 1;
+      b[i] = nd.a;
+      divlow[i] = nd.lo;
+      divhigh[i] = nd.hi;
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    bbox6[2 * d] = t.bbox[d][0];
+    bbox6[2 * d + 1] = t.bbox[d][1];
+  }
+  return VT_OK;
+}
+
+void vt_cloud_destroy(vt_cloud *c) { delete c; }
+
+size_t vt_cloud_npart(const vt_cloud *c) { return c ? c->n : 0; }
+size_t vt_cloud_nfields(const vt_cloud *c) { return c ? c->nf : 0; }
+int vt_cloud_periodic(const vt_cloud *c) { return c ? c->periodic : 0; }
+int vt_cloud_valid_rgba(const vt_cloud *c) { return c ? c->valid_rgba : 0; }
+int vt_cloud_tree_built(const vt_cloud *c) { return c ? (int)c->tree_built : 0; }
+double vt_cloud_pad(const vt_cloud *c) { return c ? c->pad : 0.0; }
+void vt_cloud_subbox(const vt_cloud *c, double out6[6]) {
+  if (c && out6) std::memcpy(out6, c->subbox, sizeof c->subbox);
+}
+void vt_cloud_orig_ids(const vt_cloud *c, int64_t *out) {
+  if (c && out) std::memcpy(out, c->orig_ids.data(), c->n * sizeof(int64_t));
+}
+
+int vt_cloud_tree_info(const vt_cloud *c, vt_tree_info *out) {
+  if (!c || !out) return fail(VT_EINVAL, "null argument");
+  if (!c->host_tree)
+    return fail(VT_ERUNTIME, "There is currently no valid tree for this point cloud");
+  out->nodes = (int64_t)c->tree.nodes.size();
+  out->leaves = c->tree.leaves;
+  out->max_depth = c->tree.max_depth;
+  out->build_seconds = c->build_seconds;
+  return VT_OK;
+}
+
+int vt_query_knn(const vt_cloud *cc, const double *d_q, size_t nq, int k, int64_t *d_ids,
+                 double *d_r2, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (rc) return rc;
+    if (k < 1 || k > kMaxQueryK) return fail(VT_EINVAL, "k must be in [1, 128]");
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (nq == 0) return VT_OK;
+    reset_call(c, st);
+    Timer tm;
+    if (k <= 3) {
+      int grid = 0;
+      if (k == 1) grid = grid_for(k_query_reg<1>, nq, c->device);
+      if (k == 2) grid = grid_for(k_query_reg<2>, nq, c->device);
+      if (k == 3) grid = grid_for(k_query_reg<3>, nq, c->device);
+      ensure_scratch(c, (size_t)grid * kBlock);
+      HIP_OK(hipEventRecord(tm.e0, st));
+      if (k == 1)
+        k_query_reg<1><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq,
+                                                c->d_slot_id, d_ids, d_r2, c->d_err,
+                                                c->d_stats);
+      if (k == 2)
+        k_query_reg<2><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq,
+                                                c->d_slot_id, d_ids, d_r2, c->d_err,
+                                                c->d_stats);
+      if (k == 3)
+        k_query_reg<3><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq,
+                                                c->d_slot_id, d_ids, d_r2, c->d_err,
+                                                c->d_stats);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipEventRecord(tm.e1, st));
+      return finish_call(c, st, tm.e0, tm.e1);
+    }
+    const int grid = grid_for(k_query_mem, nq, c->device);
+    const size_t lanes = (size_t)grid * kBlock;
+    ensure_scratch(c, lanes);
+    uint32_t *wid = dev_alloc<uint32_t>(lanes * 4 * (size_t)k);
+    double *wd = dev_alloc<double>(lanes * 4 * (size_t)k);
+    HIP_OK(hipEventRecord(tm.e0, st));
+    k_query_mem<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq, k, wid, wd,
+                                         c->d_slot_id, d_ids, d_r2, c->d_err, c->d_stats);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(tm.e1, st));
+    rc = finish_call(c, st, tm.e0, tm.e1);
+    (void)hipFree(wid);
+    (void)hipFree(wd);
+    return rc;
+  });
+}
+
+
+int vt_project(const vt_cloud *cc, const double *d_start, const double *d_end, size_t nrays,
+               int mode, double *d_out, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (!rc) rc = check_mode(cc, mode);
+    if (rc) return rc;
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (nrays == 0) return VT_OK;
+    const int nout = mode == VT_VOLUME ? 3 : (int)c->nf;
+    reset_call(c, st);
+    Timer tm;
+    const int64_t n = (int64_t)nrays;
+    if (mode == VT_SUM) {
+      switch (c->nf) {
+        case 1: launch_project<VT_SUM, 1>(c, d_start, d_end, n, d_out, nout, st, tm); break;
+        case 2: launch_project<VT_SUM, 2>(c, d_start, d_end, n, d_out, nout, st, tm); break;
+        case 3: launch_project<VT_SUM, 3>(c, d_start, d_end, n, d_out, nout, st, tm); break;
+        case 4: launch_project<VT_SUM, 4>(c, d_start, d_end, n, d_out, nout, st, tm); break;
+        default: launch_project<VT_SUM, 0>(c, d_start, d_end, n, d_out, nout, st, tm); break;
+      }
+    } else if (mode == VT_MAX) {
+      launch_project<VT_MAX, 0>(c, d_start, d_end, n, d_out, nout, st, tm);
+    } else if (mode == VT_MIN) {
+      launch_project<VT_MIN, 0>(c, d_start, d_end, n, d_out, nout, st, tm);
+    } else {
+      launch_project<VT_VOLUME, 4>(c, d_start, d_end, n, d_out, nout, st, tm);
+    }
+    return finish_call(c, st, tm.e0, tm.e1);
+  });
+}
+
+int vt_project_count(const vt_cloud *cc, const double *d_start, const double *d_end,
+                     size_t nrays, int64_t *d_counts, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (rc) return rc;
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (nrays == 0) return VT_OK;
+    reset_call(c, st);
+    Timer tm;
+    const int grid = grid_for(k_count, (int64_t)nrays, c->device);
+    ensure_scratch(c, (size_t)grid * kBlock);
+    HIP_OK(hipEventRecord(tm.e0, st));
+    k_count<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_start, d_end, (int64_t)nrays,
+                                     d_counts, c->d_err, c->d_stats);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(tm.e1, st));
+    return finish_call(c, st, tm.e0, tm.e1);
+  });
+}
+
+int vt_project_segments(const vt_cloud *cc, const double *d_start, const double *d_end,
+                        size_t nrays, int mode, const int64_t *d_offsets, double *d_out,
+                        int64_t *d_cell_ids, double *d_s_enter, double *d_s_exit,
+                        void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (!rc) rc = check_mode(cc, mode);
+    if (rc) return rc;
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (nrays == 0) return VT_OK;
+    const int nout = mode == VT_VOLUME ? 3 : (int)c->nf;
+    reset_call(c, st);
+    Timer tm;
+    const int64_t n = (int64_t)nrays;
+    auto go = [&](auto kernel) {
+      const int grid = grid_for(kernel, n, c->device);
+      ensure_scratch(c, (size_t)grid * kBlock);
+      HIP_OK(hipEventRecord(tm.e0, st));
+      kernel<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_start, d_end, n, d_offsets, d_out,
+                                      nout, c->d_slot_id, d_cell_ids, d_s_enter, d_s_exit,
+                                      c->d_err, c->d_stats);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipEventRecord(tm.e1, st));
+    };
+    if (mode == VT_SUM) go(k_fill<VT_SUM>);
+    if (mode == VT_MAX) go(k_fill<VT_MAX>);
+    if (mode == VT_MIN) go(k_fill<VT_MIN>);
+    if (mode == VT_VOLUME) go(k_fill<VT_VOLUME>);
+    return finish_call(c, st, tm.e0, tm.e1);
+  });
+}
+
+int vt_last_call_stats(vt_call_stats *out) {
+  if (!out) return fail(VT_EINVAL, "null argument");
+  *out = g_stats;
+  return VT_OK;
+}
+
+}  // extern "C"

