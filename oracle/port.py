@@ -47,8 +47,14 @@ def lib():
         L.vto_detailed.argtypes = [P, P, P, S, I, I, C.POINTER(P), C.POINTER(S)]
         L.vto_detailed_copy.argtypes = [P] * 6
         L.vto_detailed_destroy.argtypes = [P]
+        L.vto_set_mutation.argtypes = [I]
         _lib = L
     return _lib
+
+
+def set_mutation(m: int) -> None:
+    """Test-only: plant mistake m in the oracle (0 restores it)."""
+    lib().vto_set_mutation(int(m))
 
 
 def _check(rc):

@@ -39,6 +39,10 @@
 
 static __thread char g_err[512];
 
+/* Planted mistakes, for tests that show the pins catch them (0 = none). */
+static int g_mut = 0;
+void vto_set_mutation(int m) { g_mut = m; }
+
 const char *vto_last_error(void) { return g_err; }
 
 static int set_err(int code, const char *fmt, ...) {
@@ -258,7 +262,7 @@ static void rs_add(rset *r, double dist, size_t index) {
   /* KNNResultSet::addPoint without NANOFLANN_FIRST_MATCH (175-202) */
   size_t i;
   for (i = r->count; i > 0; --i) {
-    if (r->d[i - 1] > dist) {
+    if (r->d[i - 1] > dist || (g_mut == 1 && r->d[i - 1] == dist)) {
       if (i < r->k) {
         r->d[i] = r->d[i - 1];
         r->ids[i] = r->ids[i - 1];
@@ -276,6 +280,13 @@ static void rs_add(rset *r, double dist, size_t index) {
 static inline double eval_metric(const vto_cloud *c, const double *q, size_t idx) {
   /* L2_Simple_Adaptor::evalMetric (432-440): ((0 + d0^2) + d1^2) + d2^2 */
   double result = 0.0;
+  if (g_mut == 3) {
+    for (int i = 2; i >= 0; --i) {
+      double diff = q[i] - P(c, idx, i);
+      result += diff * diff;
+    }
+    return result;
+  }
   for (int i = 0; i < 3; ++i) {
     double diff = q[i] - P(c, idx, i);
     result += diff * diff;
@@ -301,7 +312,7 @@ static void search_level(const vto_cloud *c, rset *r, const double *q, int32_t n
   double diff2 = val - nd->hi;
   int32_t best, other;
   double cut_dist;
-  if ((diff1 + diff2) < 0) {
+  if ((diff1 + diff2) < 0 || (g_mut == 2 && (diff1 + diff2) == 0)) {
     best = nd->c1;
     other = nd->c2;
     cut_dist = (val - nd->hi) * (val - nd->hi);
@@ -382,13 +393,14 @@ static void query_k(const vto_cloud *c, const double *q, size_t k, size_t *res, 
     for (int iy = -1; iy <= 1; iy++)
       for (int iz = -1; iz <= 1; iz++) {
         if (ix == 0 && iy == 0 && iz == 0) continue;
+        if (g_mut == 8 && abs(ix) + abs(iy) + abs(iz) > 1) continue;
         double sh[3] = {q[0] + ix * c->box[0], q[1] + iy * c->box[1], q[2] + iz * c->box[2]};
         double minD2 = min_dist_sq_to_box(c, sh);
         if (minD2 > merged[k - 1].d + TOL) continue;
         knn_search(c, sh, k, img_ids, img_r2);
         for (size_t i = 0; i < k; i++) {
           int found = 0;
-          for (size_t j = 0; j < m; j++) {
+          for (size_t j = 0; j < m && g_mut != 4; j++) {
             if (merged[j].id == img_ids[i]) {
               merged[j].d = (img_r2[i] < merged[j].d) ? img_r2[i] : merged[j].d;
               found = 1;
@@ -497,7 +509,8 @@ static double split_analytic(const ray *r, const double *p1, const double *p2) {
   double norm[3], ppl[3];
   for (int j = 0; j < 3; j++) norm[j] = p2[j] - p1[j];
   for (int j = 0; j < 3; j++) ppl[j] = 0.5 * (p1[j] + p2[j]) - r->start[j];
-  double ndp = norm[0] * ppl[0] + norm[1] * ppl[1] + norm[2] * ppl[2];
+  double ndp = g_mut == 6 ? norm[0] * ppl[0] + (norm[1] * ppl[1] + norm[2] * ppl[2])
+                          : norm[0] * ppl[0] + norm[1] * ppl[1] + norm[2] * ppl[2];
   double ndd = norm[0] * r->dir[0] + norm[1] * r->dir[1] + norm[2] * r->dir[2];
   if (fabs(ndd) < PAR_TOL) return DBL_MAX;
   return ndp / ndd;
@@ -651,7 +664,7 @@ static int ray_walk(const vto_cloud *c, ray *r) {
   double s, pos[3];
   while (not_done) {
     if (++iteration > MAX_IT) return set_err(2, "Ray walk exceeded maximum iteration count");
-    if (r->pts[next].s - r->pts[current].s < 1e-14) {
+    if (g_mut != 9 && r->pts[next].s - r->pts[current].s < 1e-14) {
       current = next;
       ct = r->pts[current].tree_id;
       open_cell = ct;
@@ -704,7 +717,7 @@ static int ray_walk(const vto_cloud *c, ray *r) {
                          100);
       }
     }
-    if (has_c || has_n) {
+    if (has_c || (has_n && g_mut != 5)) {
       ray_push_seg(r, open_cell, entry_s, s);
       open_cell = nt;
       entry_s = s;
@@ -768,7 +781,10 @@ static int reduce(const vto_cloud *c, const ray *r, int mode, double *out) {
       for (size_t f = 0; f < nf; f++) out[f] = 0.0;
       for (size_t i = 0; i < r->nseg; i++)
         for (size_t f = 0; f < nf; f++)
-          out[f] += (r->segs[i].s_exit - r->segs[i].s_enter) * c->fields[r->segs[i].cell * nf + f];
+          out[f] += g_mut == 7 ? r->segs[i].s_exit * c->fields[r->segs[i].cell * nf + f] -
+                                     r->segs[i].s_enter * c->fields[r->segs[i].cell * nf + f]
+                               : (r->segs[i].s_exit - r->segs[i].s_enter) *
+                                     c->fields[r->segs[i].cell * nf + f];
       return 0;
     case 1:
     case 2:

@@ -26,6 +26,9 @@ typedef struct vto_cloud vto_cloud;
 /* Return codes: 0 ok, 1 invalid argument (ValueError), 2 runtime error. */
 const char *vto_last_error(void);
 
+/* Test-only: plant mistake number m (0 = none) to show the pins catch it. */
+void vto_set_mutation(int m);
+
 int vto_cloud_create(const double *pos, size_t npart, const double *fields, size_t nfields,
                      const double subbox[6], const double *vol, size_t nvol, int periodic,
                      int build_tree, vto_cloud **out);

@@ -62,6 +62,23 @@ LATTICE_RAYS = [
 ]
 
 
+def _through(p, d, back=1.3, fwd=1.7):
+    d = np.asarray(d, float) / np.linalg.norm(d)
+    p = np.asarray(p, float)
+    return (list(p - back * d), list(p + fwd * d))
+
+
+# Oblique rays through Voronoi vertices and edge midpoints: case 1a with a
+# different cell on each side (two points at one s, then a zero-width step).
+LATTICE_RAYS += [
+    _through([1.5, 1.5, 1.5], [1, 1, 0]),
+    _through([1.5, 1.5, 1.5], [1, 2, 0]),
+    _through([1.5, 1.5, 1.5], [1, 2, 3]),
+    _through([2.5, 1.5, 2.0], [1, 1, 1]),
+    _through([2.0, 2.5, 1.5], [0, 1, 2]),
+]
+
+
 def grid_cloud(box, spacing, field_val=1.0):
     """_make_grid_cloud of the reference (test_periodic.py:10-18) as arrays."""
     c = np.arange(spacing / 2.0, box, spacing)

@@ -245,3 +245,21 @@ def test_empty_and_degenerate_inputs():
                              boundbox=[0, 1, 0, 1, 0, 1])
     np.testing.assert_allclose(one.projection(np.array([[0.5, 0.5, 0.0]]),
                                               np.array([[0.5, 0.5, 1.0]])), [1.0])
+
+
+def test_knn_tiny_periodic_duplicates():
+    """A 12-point periodic cloud in a unit box: images return the same
+    particle several times and must be merged (src/pointcloud.cpp:340-353)."""
+    rng = np.random.default_rng(5)
+    pos = rng.random((12, 3))
+    bb = [0, 1, 0, 1, 0, 1]
+    pc = vt.PointCloud()
+    pc.loadPoints(pos, np.ones(12), bb, periodic=True)
+    pc.buildTree()
+    oc = port.OracleCloud(pos, np.ones(12), bb, periodic=True)
+    q = rng.random((200, 3))
+    for k in (1, 2, 3, 10):
+        ids, r2 = pc.query_tensor(torch.from_numpy(q), k)
+        oi, orr = oc.query(q, k)
+        np.testing.assert_array_equal(ids.cpu().numpy(), oi)
+        np.testing.assert_array_equal(r2.cpu().numpy(), orr)

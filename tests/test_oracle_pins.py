@@ -256,3 +256,79 @@ def test_errors_and_warnings():
     assert any("15% of longest box side" in w for w in c.warnings)
     with pytest.raises(ValueError, match="R, G, B"):
         c.project(np.zeros((1, 3)), np.ones((1, 3)), "volume")
+
+
+# Each planted mistake (oracle/vt_oracle.c, vto_set_mutation) must make at
+# least one pin fail; this shows the pins above are sharp enough.
+MUTATIONS = {
+    1: "equal r^2 inserted before earlier entries (tie order)",
+    2: "child order flipped on exact ties",
+    3: "distance summed in reverse dimension order",
+    4: "periodic merge without de-duplication",
+    5: "case 2 decided by the current cell only",
+    6: "split-point dot product associated differently",
+    7: "sum reduction as s_exit*f - s_enter*f",
+    8: "periodic search over face images only",
+    9: "zero-width step not skipped",
+}
+
+
+def _pins_hold():
+    """The pins above in compact form; False as soon as one fails."""
+    pos, f, vol, bb = cubic_lattice()
+    a, b = _clouds(pos, f, bb, vol=vol)
+    s = np.array([r[0] for r in LATTICE_RAYS], float)
+    e = np.array([r[1] for r in LATTICE_RAYS], float)
+    ra, rb = a.detailed(s, e), b.detailed(s, e)
+    if not all(np.array_equal(x, y) for x, y in zip(ra, rb)):
+        return False
+    rng = np.random.default_rng(11)
+    P = rng.random((3000, 3)) * 10
+    F = np.exp(rng.standard_normal(3000))
+    for periodic in (False, True):
+        a, b = _clouds(P, F, [0, 10, 0, 10, 0, 10], periodic=periodic)
+        gs, ge = grid.generate_projection_grid([0, 10], 12, [0, 10], None)
+        ra = a.detailed(gs.reshape(-1, 3), ge.reshape(-1, 3))
+        rb = b.detailed(gs.reshape(-1, 3), ge.reshape(-1, 3))
+        if not all(np.array_equal(x, y) for x, y in zip(ra, rb)):
+            return False
+        q = rng.random((500, 3)) * 10
+        q[:200] *= 0.05
+        for k in (3, 5):
+            ia, da = a.query(q, k)
+            ib, db = b.query(q, k)
+            if not (np.array_equal(ia, ib) and np.array_equal(da, db)):
+                return False
+    # a tiny periodic cloud: the same particle returns through several images
+    rng2 = np.random.default_rng(5)
+    P2 = rng2.random((12, 3))
+    a, b = _clouds(P2, np.ones(12), [0, 1, 0, 1, 0, 1], periodic=True)
+    q = rng2.random((50, 3))
+    if not all(np.array_equal(x, y) for x, y in zip(a.query(q, 10), b.query(q, 10))):
+        return False
+    # lattice queries on exact ties (vertices, edge midpoints)
+    pos, f, vol, bb = cubic_lattice()
+    a, b = _clouds(pos, f, bb, vol=vol)
+    q = np.array([[0.5, 0.5, 0.5], [1.5, 2.0, 2.5], [2.0, 2.5, 2.5], [3.5, 3.5, 0.5]])
+    for k in (1, 3, 6):
+        if not all(np.array_equal(x, y) for x, y in zip(a.query(q, k), b.query(q, k))):
+            return False
+    return True
+
+
+@needs_ref
+def test_pins_hold_unmutated():
+    assert _pins_hold()
+
+
+@needs_ref
+@pytest.mark.parametrize("m", sorted(MUTATIONS))
+def test_planted_mistake_is_caught(m):
+    port.set_mutation(m)
+    try:
+        caught = not _pins_hold()
+    except (RuntimeError, ValueError):
+        caught = True
+    finally:
+        port.set_mutation(0)
+    assert caught, f"mutation {m} ({MUTATIONS[m]}) passed every pin"

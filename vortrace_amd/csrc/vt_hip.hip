@@ -18,6 +18,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -455,18 +456,37 @@ struct vt_cloud {
 namespace {
 
 int num_cus(int dev) {
-  hipDeviceProp_t p;
-  HIP_OK(hipGetDeviceProperties(&p, dev));
-  return p.multiProcessorCount;
+  static std::mutex mu;
+  static std::vector<int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+  if (!cache[dev]) {
+    hipDeviceProp_t p;
+    HIP_OK(hipGetDeviceProperties(&p, dev));
+    cache[dev] = p.multiProcessorCount;
+  }
+  return cache[dev];
 }
 
 // Persistent grid size for a kernel: as many workgroups as fit on the device
-// at once, but no more than the work needs.
+// at once, but no more than the work needs.  Occupancy is queried once per
+// kernel.
 template <class K>
 int grid_for(K kernel, int64_t items, int dev) {
+  static std::mutex mu;
+  static std::map<const void *, int> cache;
   int per_cu = 0;
-  HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
-  if (per_cu < 1) per_cu = 1;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(reinterpret_cast<const void *>(kernel));
+    if (it != cache.end()) per_cu = it->second;
+  }
+  if (!per_cu) {
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    if (per_cu < 1) per_cu = 1;
+    std::lock_guard<std::mutex> lk(mu);
+    cache[reinterpret_cast<const void *>(kernel)] = per_cu;
+  }
   int64_t full = (int64_t)per_cu * num_cus(dev);
   int64_t need = (items + kBlock - 1) / kBlock;
   return (int)std::max<int64_t>(1, std::min(full, need));
