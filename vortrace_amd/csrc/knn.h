@@ -264,33 +264,60 @@ __device__ void knn_core(const DevCloud &C, const double q0, const double q1, co
   }
 }
 
-// PointCloud::minDistSqToBox, src/pointcloud.cpp:283-298.
-__device__ __forceinline__ double min_dist_sq_to_box(const DevCloud &C, const double *q) {
-  double d2 = 0.0;
+// PointCloud::minDistSqToBox (src/pointcloud.cpp:283-298) for the 27 shifted
+// queries q + i*L, factored per dimension.  The reference adds, in dimension
+// order, (lo - v)^2 or (v - hi)^2 for every coordinate v outside [lo, hi] to
+// d2 = 0; adding 0.0 for an inside coordinate changes no bits, so
+// d2(ix, iy, iz) = ((c[0][ix] + c[1][iy]) + c[2][iz]) is bit-identical, with
+// c[d][s] computed once per query from the same shifted coordinate.
+struct ImageBox {
+  double c[3][3];  // [dim][shift + 1]
+  __device__ __forceinline__ void init(const DevCloud &C, const double *q) {
 #pragma unroll
-  for (int dim = 0; dim < 3; ++dim) {
-    const double lo = C.sb[2 * dim], hi = C.sb[2 * dim + 1];
-    if (q[dim] < lo) {
-      const double d = lo - q[dim];
-      d2 += d * d;
-    } else if (q[dim] > hi) {
-      const double d = q[dim] - hi;
-      d2 += d * d;
+    for (int d = 0; d < 3; ++d) {
+      const double lo = C.sb[2 * d], hi = C.sb[2 * d + 1];
+#pragma unroll
+      for (int s = -1; s <= 1; ++s) {
+        const double v = q[d] + s * C.box[d];
+        double e = 0.0;
+        if (v < lo) {
+          const double t = lo - v;
+          e = t * t;
+        } else if (v > hi) {
+          const double t = v - hi;
+          e = t * t;
+        }
+        c[d][s + 1] = e;
+      }
     }
   }
-  return d2;
-}
+  __device__ __forceinline__ double d2(int ix, int iy, int iz) const {
+    return (c[0][ix + 1] + c[1][iy + 1]) + c[2][iz + 1];
+  }
+  // False when every image has a coordinate whose own contribution already
+  // exceeds w; sums of non-negative terms round monotonically, so each such
+  // image's d2 exceeds w too and the reference prunes all of them.
+  __device__ __forceinline__ bool any_within(double w) const {
+    bool any = false;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) any |= (c[d][0] <= w) | (c[d][2] <= w);
+    // an image shifted in one dimension only also needs the others' 0-shift
+    // terms small; those are 0 for a query inside the box and checked per
+    // image otherwise.
+    return any;
+  }
+};
 
 // PointCloud::queryTreeK for k = K <= 3 (src/pointcloud.cpp:308-368): the
 // plain search, then in periodic mode up to 26 image searches merged with
 // de-duplication by id (keeping the smaller r^2), a stable sort by r^2 and
 // truncation to K after every image.
-template <int K, class ST>
+template <int K, bool kPeriodic, class ST>
 __device__ void query_reg(const DevCloud &C, const double q0, const double q1, const double q2,
                           uint32_t (&ids)[K], double (&r2)[K], ST &st) {
   RegRes<K> rs;
   knn_core(C, q0, q1, q2, rs, st);
-  if (!C.periodic) {
+  if constexpr (!kPeriodic) {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       ids[j] = rs.id[j];
@@ -312,14 +339,23 @@ __device__ void query_reg(const DevCloud &C, const double q0, const double q1, c
     md[j] = DBL_MAX;
   }
   const double qv[3] = {q0, q1, q2};
+  ImageBox ib;
+  ib.init(C, qv);
+  if (!ib.any_within(md[K - 1] + kTol)) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      ids[j] = mi[j];
+      r2[j] = md[j];
+    }
+    return;
+  }
   for (int ix = -1; ix <= 1; ix++)
     for (int iy = -1; iy <= 1; iy++)
       for (int iz = -1; iz <= 1; iz++) {
         if (ix == 0 && iy == 0 && iz == 0) continue;
+        if (ib.d2(ix, iy, iz) > md[K - 1] + kTol) continue;
         const double sh[3] = {qv[0] + ix * C.box[0], qv[1] + iy * C.box[1],
                               qv[2] + iz * C.box[2]};
-        const double minD2 = min_dist_sq_to_box(C, sh);
-        if (minD2 > md[K - 1] + kTol) continue;
         RegRes<K> ri;
         knn_core(C, sh[0], sh[1], sh[2], ri, st);
 #pragma unroll
@@ -382,14 +418,15 @@ __device__ void query_mem(const DevCloud &C, const double q0, const double q1, c
   }
   MemRes img{res.d + k, res.id + k, k, 0};  // image results after the k slots
   const double qv[3] = {q0, q1, q2};
+  ImageBox ib;
+  ib.init(C, qv);
   for (int ix = -1; ix <= 1; ix++)
     for (int iy = -1; iy <= 1; iy++)
       for (int iz = -1; iz <= 1; iz++) {
         if (ix == 0 && iy == 0 && iz == 0) continue;
+        if (ib.d2(ix, iy, iz) > md[k - 1] + kTol) continue;
         const double sh[3] = {qv[0] + ix * C.box[0], qv[1] + iy * C.box[1],
                               qv[2] + iz * C.box[2]};
-        const double minD2 = min_dist_sq_to_box(C, sh);
-        if (minD2 > md[k - 1] + kTol) continue;
         knn_core(C, sh[0], sh[1], sh[2], img, st);
         for (int i = 0; i < k; ++i) {
           bool found = false;

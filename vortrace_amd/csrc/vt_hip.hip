@@ -38,7 +38,8 @@ constexpr int kLdsDepth = 16;  // kNN stack entries per lane held in LDS
 constexpr int kWalkCap = 128;  // pending ray points per lane (device memory)
 
 using Stack = KStack<kLdsDepth, kBlock>;
-using Ctx = WalkCtx<kLdsDepth, kBlock>;
+template <bool P>
+using Ctx = WalkCtx<kLdsDepth, kBlock, P>;
 
 // ---------------------------------------------------------------- sinks ----
 
@@ -173,21 +174,21 @@ __device__ __forceinline__ void flush_stats(Stats *stats, uint64_t nseg, uint64_
   atomicMax(&stats->max_knn_depth, (unsigned int)maxk);
 }
 
-#define VT_LANE_SETUP                                                                      \
-  __shared__ uint32_t lds_tag[kLdsDepth * kBlock];                                         \
-  __shared__ double lds_val[kLdsDepth * kBlock];                                           \
-  const int tid = threadIdx.x;                                                             \
-  const int gtid = blockIdx.x * kBlock + tid;                                              \
-  Ctx x{C, Stack{lds_tag + tid, lds_val + tid, S.knn_tag + gtid, S.knn_val + gtid, S.lanes, \
-                 S.kspill, 0, 0, false},                                                   \
-        S, gtid, (int)(tid & 63), gtid >> 6, 0, 0, false};
+#define VT_LANE_SETUP(PER)                                                                  \
+  __shared__ uint32_t lds_tag[kLdsDepth * kBlock];                                          \
+  __shared__ double lds_val[kLdsDepth * kBlock];                                            \
+  const int tid = threadIdx.x;                                                              \
+  const int gtid = blockIdx.x * kBlock + tid;                                               \
+  Ctx<PER> x{C, Stack{lds_tag + tid, lds_val + tid, S.knn_tag + gtid, S.knn_val + gtid,     \
+                      S.lanes, S.kspill, 0, 0, false},                                      \
+             S, gtid, (int)(tid & 63), gtid >> 6, 0, 0, false};
 
-template <int MODE, int NF>
+template <int MODE, int NF, bool PER>
 __global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const double *start,
                                                     const double *end, int64_t nrays,
                                                     double *out, int nout, ErrRec *err,
                                                     Stats *stats) {
-  VT_LANE_SETUP
+  VT_LANE_SETUP(PER)
   uint64_t nseg = 0;
   int maxw = 0;
   for (;;) {
@@ -229,10 +230,11 @@ __global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const
   flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
 }
 
+template <bool PER>
 __global__ __launch_bounds__(kBlock) void k_count(DevCloud C, Scratch S, const double *start,
                                                   const double *end, int64_t nrays,
                                                   int64_t *counts, ErrRec *err, Stats *stats) {
-  VT_LANE_SETUP
+  VT_LANE_SETUP(PER)
   uint64_t nseg = 0;
   int maxw = 0;
   for (;;) {
@@ -254,14 +256,14 @@ __global__ __launch_bounds__(kBlock) void k_count(DevCloud C, Scratch S, const d
   flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
 }
 
-template <int MODE>
+template <int MODE, bool PER>
 __global__ __launch_bounds__(kBlock) void k_fill(DevCloud C, Scratch S, const double *start,
                                                  const double *end, int64_t nrays,
                                                  const int64_t *offsets, double *out, int nout,
                                                  const uint32_t *slot_id, int64_t *ids,
                                                  double *se, double *sx, ErrRec *err,
                                                  Stats *stats) {
-  VT_LANE_SETUP
+  VT_LANE_SETUP(PER)
   uint64_t nseg = 0;
   int maxw = 0;
   for (;;) {
@@ -306,18 +308,18 @@ __global__ __launch_bounds__(kBlock) void k_fill(DevCloud C, Scratch S, const do
   flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
 }
 
-template <int K>
+template <int K, bool PER>
 __global__ __launch_bounds__(kBlock) void k_query_reg(DevCloud C, Scratch S, const double *q,
                                                       int64_t nq, const uint32_t *slot_id,
                                                       int64_t *ids, double *r2, ErrRec *err,
                                                       Stats *stats) {
-  VT_LANE_SETUP
+  VT_LANE_SETUP(PER)
   for (;;) {
     const int64_t i = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
     if (i >= nq) break;
     uint32_t id[K];
     double d[K];
-    query_reg<K>(C, q[3 * i], q[3 * i + 1], q[3 * i + 2], id, d, x.st);
+    query_reg<K, PER>(C, q[3 * i], q[3 * i + 1], q[3 * i + 2], id, d, x.st);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       ids[i * K + j] = id[j] == kNone ? -1 : (int64_t)slot_id[id[j]];
@@ -328,12 +330,13 @@ __global__ __launch_bounds__(kBlock) void k_query_reg(DevCloud C, Scratch S, con
   flush_stats(stats, 0, 0, 0, 0, x.st.maxsp);
 }
 
+template <bool PER>
 __global__ __launch_bounds__(kBlock) void k_query_mem(DevCloud C, Scratch S, const double *q,
                                                       int64_t nq, int k, uint32_t *wid,
                                                       double *wd, const uint32_t *slot_id,
                                                       int64_t *ids, double *r2, ErrRec *err,
                                                       Stats *stats) {
-  VT_LANE_SETUP
+  VT_LANE_SETUP(PER)
   uint32_t *bi = wid + (size_t)gtid * 4 * k;
   double *bd = wd + (size_t)gtid * 4 * k;
   for (;;) {
@@ -615,16 +618,25 @@ int guarded(F &&f) {
 
 namespace {
 
+template <int MODE, int NF, bool PER>
+void launch_project_p(vt_cloud *c, const double *s, const double *e, int64_t n, double *out,
+                      int nout, hipStream_t st, Timer &tm) {
+  const int grid = grid_for(k_project<MODE, NF, PER>, n, c->device);
+  ensure_scratch(c, (size_t)grid * kBlock);
+  HIP_OK(hipEventRecord(tm.e0, st));
+  k_project<MODE, NF, PER><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, s, e, n, out, nout,
+                                                    c->d_err, c->d_stats);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(tm.e1, st));
+}
+
 template <int MODE, int NF>
 void launch_project(vt_cloud *c, const double *s, const double *e, int64_t n, double *out,
                     int nout, hipStream_t st, Timer &tm) {
-  const int grid = grid_for(k_project<MODE, NF>, n, c->device);
-  ensure_scratch(c, (size_t)grid * kBlock);
-  HIP_OK(hipEventRecord(tm.e0, st));
-  k_project<MODE, NF><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, s, e, n, out, nout, c->d_err,
-                                               c->d_stats);
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipEventRecord(tm.e1, st));
+  if (c->periodic)
+    launch_project_p<MODE, NF, true>(c, s, e, n, out, nout, st, tm);
+  else
+    launch_project_p<MODE, NF, false>(c, s, e, n, out, nout, st, tm);
 }
 
 int check_mode(const vt_cloud *c, int mode) {
@@ -902,36 +914,35 @@ int vt_query_knn(const vt_cloud *cc, const double *d_q, size_t nq, int k, int64_
     reset_call(c, st);
     Timer tm;
     if (k <= 3) {
-      int grid = 0;
-      if (k == 1) grid = grid_for(k_query_reg<1>, nq, c->device);
-      if (k == 2) grid = grid_for(k_query_reg<2>, nq, c->device);
-      if (k == 3) grid = grid_for(k_query_reg<3>, nq, c->device);
-      ensure_scratch(c, (size_t)grid * kBlock);
-      HIP_OK(hipEventRecord(tm.e0, st));
-      if (k == 1)
-        k_query_reg<1><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq,
-                                                c->d_slot_id, d_ids, d_r2, c->d_err,
-                                                c->d_stats);
-      if (k == 2)
-        k_query_reg<2><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq,
-                                                c->d_slot_id, d_ids, d_r2, c->d_err,
-                                                c->d_stats);
-      if (k == 3)
-        k_query_reg<3><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq,
-                                                c->d_slot_id, d_ids, d_r2, c->d_err,
-                                                c->d_stats);
-      HIP_OK(hipGetLastError());
-      HIP_OK(hipEventRecord(tm.e1, st));
+      auto go = [&](auto kernel) {
+        const int grid = grid_for(kernel, nq, c->device);
+        ensure_scratch(c, (size_t)grid * kBlock);
+        HIP_OK(hipEventRecord(tm.e0, st));
+        kernel<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq, c->d_slot_id,
+                                        d_ids, d_r2, c->d_err, c->d_stats);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipEventRecord(tm.e1, st));
+      };
+      if (c->periodic) {
+        if (k == 1) go(k_query_reg<1, true>);
+        if (k == 2) go(k_query_reg<2, true>);
+        if (k == 3) go(k_query_reg<3, true>);
+      } else {
+        if (k == 1) go(k_query_reg<1, false>);
+        if (k == 2) go(k_query_reg<2, false>);
+        if (k == 3) go(k_query_reg<3, false>);
+      }
       return finish_call(c, st, tm.e0, tm.e1);
     }
-    const int grid = grid_for(k_query_mem, nq, c->device);
+    auto kmem = c->periodic ? k_query_mem<true> : k_query_mem<false>;
+    const int grid = grid_for(kmem, nq, c->device);
     const size_t lanes = (size_t)grid * kBlock;
     ensure_scratch(c, lanes);
     uint32_t *wid = dev_alloc<uint32_t>(lanes * 4 * (size_t)k);
     double *wd = dev_alloc<double>(lanes * 4 * (size_t)k);
     HIP_OK(hipEventRecord(tm.e0, st));
-    k_query_mem<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq, k, wid, wd,
-                                         c->d_slot_id, d_ids, d_r2, c->d_err, c->d_stats);
+    kmem<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_q, (int64_t)nq, k, wid, wd,
+                                  c->d_slot_id, d_ids, d_r2, c->d_err, c->d_stats);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(tm.e1, st));
     rc = finish_call(c, st, tm.e0, tm.e1);
@@ -984,11 +995,12 @@ int vt_project_count(const vt_cloud *cc, const double *d_start, const double *d_
     if (nrays == 0) return VT_OK;
     reset_call(c, st);
     Timer tm;
-    const int grid = grid_for(k_count, (int64_t)nrays, c->device);
+    auto kc = c->periodic ? k_count<true> : k_count<false>;
+    const int grid = grid_for(kc, (int64_t)nrays, c->device);
     ensure_scratch(c, (size_t)grid * kBlock);
     HIP_OK(hipEventRecord(tm.e0, st));
-    k_count<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_start, d_end, (int64_t)nrays,
-                                     d_counts, c->d_err, c->d_stats);
+    kc<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_start, d_end, (int64_t)nrays, d_counts,
+                                c->d_err, c->d_stats);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord(tm.e1, st));
     return finish_call(c, st, tm.e0, tm.e1);
@@ -1020,10 +1032,17 @@ int vt_project_segments(const vt_cloud *cc, const double *d_start, const double 
       HIP_OK(hipGetLastError());
       HIP_OK(hipEventRecord(tm.e1, st));
     };
-    if (mode == VT_SUM) go(k_fill<VT_SUM>);
-    if (mode == VT_MAX) go(k_fill<VT_MAX>);
-    if (mode == VT_MIN) go(k_fill<VT_MIN>);
-    if (mode == VT_VOLUME) go(k_fill<VT_VOLUME>);
+    if (c->periodic) {
+      if (mode == VT_SUM) go(k_fill<VT_SUM, true>);
+      if (mode == VT_MAX) go(k_fill<VT_MAX, true>);
+      if (mode == VT_MIN) go(k_fill<VT_MIN, true>);
+      if (mode == VT_VOLUME) go(k_fill<VT_VOLUME, true>);
+    } else {
+      if (mode == VT_SUM) go(k_fill<VT_SUM, false>);
+      if (mode == VT_MAX) go(k_fill<VT_MAX, false>);
+      if (mode == VT_MIN) go(k_fill<VT_MIN, false>);
+      if (mode == VT_VOLUME) go(k_fill<VT_VOLUME, false>);
+    }
     return finish_call(c, st, tm.e0, tm.e1);
   });
 }

@@ -34,8 +34,9 @@ __device__ __forceinline__ void make_ray(const double *s, const double *e, RayGe
   r.len = L;
 }
 
-template <int kLdsDepth, int kBlock>
+template <int kLdsDepth, int kBlock, bool kPer>
 struct WalkCtx {
+  static constexpr bool kPeriodic = kPer;
   const DevCloud &C;
   KStack<kLdsDepth, kBlock> st;
   const Scratch &S;
@@ -59,7 +60,7 @@ struct WalkCtx {
   __device__ __forceinline__ uint32_t query1(const double *q) {
     uint32_t id[1];
     double r2[1];
-    query_reg<1>(C, q[0], q[1], q[2], id, r2, st);
+    query_reg<1, kPer>(C, q[0], q[1], q[2], id, r2, st);
     ++nq;
     return guard(id[0]);
   }
@@ -90,7 +91,7 @@ __device__ int split_point(X &x, const RayGeom &r, uint32_t id1, uint32_t id2, d
   const Gen g2 = C.gen[id2];
   double p1[3] = {g1.x, g1.y, g1.z};
   double p2[3] = {g2.x, g2.y, g2.z};
-  if (C.periodic) {
+  if constexpr (X::kPeriodic) {
     const double s_ref = 0.5 * (s_lo + s_hi);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -180,7 +181,7 @@ __device__ uint32_t try_along_ray(X &x, const RayGeom &r, const double *pos, dou
   for (int j = 0; j < 3; ++j) pp[j] = pos[j] + sign * kPertEps * r.d[j];
   uint32_t ids[2];
   double r2[2];
-  query_reg<2>(x.C, pp[0], pp[1], pp[2], ids, r2, x.st);
+  query_reg<2, X::kPeriodic>(x.C, pp[0], pp[1], pp[2], ids, r2, x.st);
   ++x.nq;
   ids[0] = x.guard(ids[0]);
   if (r2[0] + kTol < r2[1] && ids[0] != ct && ids[0] != nt) return ids[0];
@@ -252,7 +253,7 @@ __device__ int walk_ray(X &x, const RayGeom &r, Sink &sink, int &max_depth, doub
   };
 
   if (c0 == c1) {
-    if (!C.periodic) {
+    if constexpr (!X::kPeriodic) {
       sink.seg(c0, 0.0, r.len);
       return kErrNone;
     }
@@ -316,7 +317,7 @@ __device__ int walk_ray(X &x, const RayGeom &r, Sink &sink, int &max_depth, doub
 
     uint32_t ids[3];
     double r2[3];
-    query_reg<3>(C, pos[0], pos[1], pos[2], ids, r2, x.st);
+    query_reg<3, X::kPeriodic>(C, pos[0], pos[1], pos[2], ids, r2, x.st);
     ++x.nq;
     ids[0] = x.guard(ids[0]);
     if (x.bad) return kErrNonFinite;
