@@ -31,13 +31,13 @@ METRIC = ("cell-crossings/s + rays/s, fwd vs fwd+adjoint, 2M clustered cells, "
           "512² image")
 
 WORKLOADS = {
-    "c1": dict(desc="100k uniform points in [0,100)^3 (seed 1), non-periodic, "
+    "uniform100k": dict(desc="100k uniform points in [0,100)^3 (seed 1), non-periodic, "
                     "128^2 z-rays over [0,100], sum",
                n=100_000, npix=128, periodic=False, mode="sum"),
-    "c2": dict(desc="2M clustered periodic mesh (box 100, seed 2), 512^2 z-rays "
+    "clustered2m": dict(desc="2M clustered periodic mesh (box 100, seed 2), 512^2 z-rays "
                     "over [0,100], sum",
                n=2_000_000, npix=512, periodic=True, mode="sum"),
-    "c3": dict(desc="2M clustered periodic mesh (seed 2), RGBA transfer function, "
+    "clustered2m_volume": dict(desc="2M clustered periodic mesh (seed 2), RGBA transfer function, "
                     "1024^2 z-rays over [0,100], volume rendering",
                n=2_000_000, npix=1024, periodic=True, mode="volume"),
 }
@@ -51,7 +51,7 @@ def make_workload(name):
     from vortrace_amd import grid, synthetic
     w = WORKLOADS[name]
     t0 = time.perf_counter()
-    if name == "c1":
+    if name == "uniform100k":
         pos, fields = synthetic.uniform_mesh(w["n"], seed=1)
     else:
         pos, fields = synthetic.clustered_mesh(w["n"], seed=2)
@@ -117,11 +117,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="clustered2m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-sample", type=float, default=0.25,
                     help="fraction of the rays timed on the CPU reference")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device-to-device copy of this many GiB "
+                         "(reference size for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+        n = int(args.calibrate_copy * (1 << 30)) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 8 * n, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "rays are independent, so N GPUs would each take 1/N of the rays.",

@@ -92,10 +92,28 @@ struct Scratch {
 
 constexpr int kMaxQueryK = 128;
 
+// Loads and stores through the global address space.  The data pointers
+// travel inside structs, where the compiler cannot see that they are global
+// and would otherwise emit flat accesses.
+template <class T>
+__device__ __forceinline__ T gld(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)(p);
+}
+template <class T>
+__device__ __forceinline__ void gst(T *p, T v) {
+  *(__attribute__((address_space(1))) T *)(p) = v;
+}
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+// 16-byte global load of two doubles (one dwordx4).
+__device__ __forceinline__ double2 gld(const double2 *p) {
+  const v2f64 v = *(const __attribute__((address_space(1))) v2f64 *)(p);
+  return make_double2(v.x, v.y);
+}
+
 __device__ __forceinline__ Node load_node(const Node *nodes, uint32_t i) {
   const double2 *p = reinterpret_cast<const double2 *>(nodes + i);
-  const double2 v0 = p[0];
-  const double2 v1 = p[1];
+  const double2 v0 = gld(p);
+  const double2 v1 = gld(p + 1);
   Node n;
   n.lo = v0.x;
   n.hi = v0.y;

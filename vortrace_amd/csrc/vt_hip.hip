@@ -33,13 +33,7 @@
 
 namespace vt {
 
-constexpr int kBlock = 128;    // lanes per workgroup (2 waves)
-constexpr int kLdsDepth = 16;  // kNN stack entries per lane held in LDS
 constexpr int kWalkCap = 128;  // pending ray points per lane (device memory)
-
-using Stack = KStack<kLdsDepth, kBlock>;
-template <bool P>
-using Ctx = WalkCtx<kLdsDepth, kBlock, P>;
 
 // ---------------------------------------------------------------- sinks ----
 
@@ -56,12 +50,12 @@ struct SumSink {  // reduce_sum, src/reduction.cpp:8-21
   __device__ __forceinline__ void seg(uint32_t cell, double a, double b) {
     const double ds = b - a;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) acc[f] = acc[f] + ds * fields[(size_t)cell * NF + f];
+    for (int f = 0; f < NF; ++f) acc[f] = acc[f] + ds * gld(fields + (size_t)cell * NF + f);
     ++n;
   }
   __device__ void store(double *out) const {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) out[f] = acc[f];
+    for (int f = 0; f < NF; ++f) gst(out + f, acc[f]);
   }
 };
 
@@ -71,12 +65,13 @@ struct SumSinkDyn {  // reduce_sum for any number of fields, accumulated in plac
   double *out;
   uint64_t n;
   __device__ void init() {
-    for (int f = 0; f < nf; ++f) out[f] = 0.0;
+    for (int f = 0; f < nf; ++f) gst(out + f, 0.0);
     n = 0;
   }
   __device__ void seg(uint32_t cell, double a, double b) {
     const double ds = b - a;
-    for (int f = 0; f < nf; ++f) out[f] = out[f] + ds * fields[(size_t)cell * nf + f];
+    for (int f = 0; f < nf; ++f)
+      gst(out + f, gld(out + f) + ds * gld(fields + (size_t)cell * nf + f));
     ++n;
   }
   __device__ void store(double *) const {}
@@ -89,13 +84,14 @@ struct ExtSink {  // reduce_max / reduce_min, src/reduction.cpp:23-53
   double *out;
   uint64_t n;
   __device__ void init() {
-    for (int f = 0; f < nf; ++f) out[f] = kMax ? -INFINITY : INFINITY;
+    for (int f = 0; f < nf; ++f) gst(out + f, kMax ? -(double)INFINITY : (double)INFINITY);
     n = 0;
   }
   __device__ void seg(uint32_t cell, double, double) {
     for (int f = 0; f < nf; ++f) {
-      const double v = fields[(size_t)cell * nf + f];
-      if (kMax ? (v > out[f]) : (v < out[f])) out[f] = v;
+      const double v = gld(fields + (size_t)cell * nf + f);
+      const double cur = gld(out + f);
+      if (kMax ? (v > cur) : (v < cur)) gst(out + f, v);
     }
     ++n;
   }
@@ -112,19 +108,21 @@ struct VolSink {  // reduce_volume_render, src/reduction.cpp:55-83
     n = 0;
   }
   __device__ __forceinline__ void seg(uint32_t cell, double a, double b) {
-    const double *fv = fields + (size_t)cell * 4;
-    const double alpha = fv[3];
+    const double2 *fv = reinterpret_cast<const double2 *>(fields + (size_t)cell * 4);
+    const double2 rg = gld(fv), ba = gld(fv + 1);
+    const double col[3] = {rg.x, rg.y, ba.x};
+    const double alpha = ba.y;
     const double ds = b - a;
     const double opacity = 1.0 - exp(-alpha * ds);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) c[k] += T * fv[k] * opacity;
+    for (int k = 0; k < 3; ++k) c[k] += T * col[k] * opacity;
     T *= (1.0 - opacity);
     ++n;
   }
   __device__ void store(double *out) const {
-    out[0] = c[0];
-    out[1] = c[1];
-    out[2] = c[2];
+    gst(out, c[0]);
+    gst(out + 1, c[1]);
+    gst(out + 2, c[2]);
   }
 };
 
@@ -149,9 +147,9 @@ struct FillSink {  // segments at the ray's offset, plus the reduced values
   }
   __device__ __forceinline__ void seg(uint32_t cell, double a, double b) {
     if (n < cap) {
-      ids[n] = (int64_t)slot_id[cell];
-      se[n] = a;
-      sx[n] = b;
+      gst(ids + n, (int64_t)gld(slot_id + cell));
+      gst(se + n, a);
+      gst(sx + n, b);
     }
     ++n;
     in.seg(cell, a, b);
@@ -174,14 +172,12 @@ __device__ __forceinline__ void flush_stats(Stats *stats, uint64_t nseg, uint64_
   atomicMax(&stats->max_knn_depth, (unsigned int)maxk);
 }
 
-#define VT_LANE_SETUP(PER)                                                                  \
-  __shared__ uint32_t lds_tag[kLdsDepth * kBlock];                                          \
-  __shared__ double lds_val[kLdsDepth * kBlock];                                            \
-  const int tid = threadIdx.x;                                                              \
-  const int gtid = blockIdx.x * kBlock + tid;                                               \
-  Ctx<PER> x{C, Stack{lds_tag + tid, lds_val + tid, S.knn_tag + gtid, S.knn_val + gtid,     \
-                      S.lanes, S.kspill, 0, 0, false},                                      \
-             S, gtid, (int)(tid & 63), gtid >> 6, 0, 0, false};
+#define VT_LANE_SETUP(PER)                                                      \
+  const int tid = threadIdx.x;                                                  \
+  const int gtid = blockIdx.x * kBlock + tid;                                   \
+  const LaneInfo lane_info{tid, gtid, gtid >> 6, (int)(tid & 63)};              \
+  KStack st = make_stack(S, lane_info);                                         \
+  WalkCounters cnt;
 
 template <int MODE, int NF, bool PER>
 __global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const double *start,
@@ -204,30 +200,30 @@ __global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const
     if constexpr (MODE == VT_SUM && NF > 0) {
       SumSink<NF> sk{C.fields};
       sk.init();
-      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
       sk.store(o);
       nseg += sk.n;
     } else if constexpr (MODE == VT_SUM) {
       SumSinkDyn sk{C.fields, (int)C.nf, o};
       sk.init();
-      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
       nseg += sk.n;
     } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
       ExtSink<MODE == VT_MAX> sk{C.fields, (int)C.nf, o};
       sk.init();
-      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
       nseg += sk.n;
     } else {
       VolSink sk{C.fields};
       sk.init();
-      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
       sk.store(o);
       nseg += sk.n;
     }
-    if (!rc && x.st.overflow) rc = kErrKnnStack;
+    if (!rc && st.overflow) rc = kErrKnnStack;
     if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
   }
-  flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
+  flush_stats(stats, nseg, cnt.nq, cnt.nslow, maxw, st.maxsp);
 }
 
 template <bool PER>
@@ -247,13 +243,13 @@ __global__ __launch_bounds__(kBlock) void k_count(DevCloud C, Scratch S, const d
     uint32_t ei[2] = {0, 0};
     CountSink sk;
     sk.init();
-    int rc = walk_ray(x, r, sk, maxw, ev, ei);
-    if (!rc && x.st.overflow) rc = kErrKnnStack;
-    counts[ray] = (int64_t)sk.n;
+    int rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
+    if (!rc && st.overflow) rc = kErrKnnStack;
+    gst(counts + ray, (int64_t)sk.n);
     nseg += sk.n;
     if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
   }
-  flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
+  flush_stats(stats, nseg, cnt.nq, cnt.nslow, maxw, st.maxsp);
 }
 
 template <int MODE, bool PER>
@@ -274,8 +270,8 @@ __global__ __launch_bounds__(kBlock) void k_fill(DevCloud C, Scratch S, const do
     make_ray(start + 3 * ray, end + 3 * ray, r);
     double ev[3] = {0, 0, 0};
     uint32_t ei[2] = {0, 0};
-    const int64_t off = offsets[ray];
-    const int64_t cap = offsets[ray + 1] - off;
+    const int64_t off = gld(offsets + ray);
+    const int64_t cap = gld(offsets + ray + 1) - off;
     double *o = out + ray * nout;
     int rc;
     uint64_t n;
@@ -283,29 +279,29 @@ __global__ __launch_bounds__(kBlock) void k_fill(DevCloud C, Scratch S, const do
       FillSink<SumSinkDyn> sk{SumSinkDyn{C.fields, (int)C.nf, o}, slot_id, ids + off, se + off,
                               sx + off, (uint64_t)cap};
       sk.init();
-      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
       n = sk.n;
     } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
       FillSink<ExtSink<MODE == VT_MAX>> sk{ExtSink<MODE == VT_MAX>{C.fields, (int)C.nf, o},
                                            slot_id, ids + off, se + off, sx + off,
                                            (uint64_t)cap};
       sk.init();
-      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
       n = sk.n;
     } else {
       FillSink<VolSink> sk{VolSink{C.fields}, slot_id, ids + off, se + off, sx + off,
                            (uint64_t)cap};
       sk.init();
-      rc = walk_ray(x, r, sk, maxw, ev, ei);
+      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
       sk.store(o);
       n = sk.n;
     }
     nseg += n;
-    if (!rc && x.st.overflow) rc = kErrKnnStack;
+    if (!rc && st.overflow) rc = kErrKnnStack;
     if (!rc && (int64_t)n != cap) rc = kErrFillOverflow;
     if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
   }
-  flush_stats(stats, nseg, x.nq, x.nslow, maxw, x.st.maxsp);
+  flush_stats(stats, nseg, cnt.nq, cnt.nslow, maxw, st.maxsp);
 }
 
 template <int K, bool PER>
@@ -319,15 +315,15 @@ __global__ __launch_bounds__(kBlock) void k_query_reg(DevCloud C, Scratch S, con
     if (i >= nq) break;
     uint32_t id[K];
     double d[K];
-    query_reg<K, PER>(C, q[3 * i], q[3 * i + 1], q[3 * i + 2], id, d, x.st);
+    query_reg<K, PER>(C, gld(q + 3 * i), gld(q + 3 * i + 1), gld(q + 3 * i + 2), id, d, st);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-      ids[i * K + j] = id[j] == kNone ? -1 : (int64_t)slot_id[id[j]];
-      r2[i * K + j] = d[j];
+      gst(ids + i * K + j, id[j] == kNone ? (int64_t)-1 : (int64_t)gld(slot_id + id[j]));
+      gst(r2 + i * K + j, d[j]);
     }
   }
-  if (x.st.overflow) report_error(err, kErrKnnStack, 0, 0, 0, 0, 0, 0);
-  flush_stats(stats, 0, 0, 0, 0, x.st.maxsp);
+  if (st.overflow) report_error(err, kErrKnnStack, 0, 0, 0, 0, 0, 0);
+  flush_stats(stats, 0, 0, 0, 0, st.maxsp);
 }
 
 template <bool PER>
@@ -343,14 +339,16 @@ __global__ __launch_bounds__(kBlock) void k_query_mem(DevCloud C, Scratch S, con
     const int64_t i = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
     if (i >= nq) break;
     MemRes res{bd, bi, k, 0};
-    query_mem(C, q[3 * i], q[3 * i + 1], q[3 * i + 2], k, res, bi + 2 * k, bd + 2 * k, x.st);
+    query_mem<PER>(C, gld(q + 3 * i), gld(q + 3 * i + 1), gld(q + 3 * i + 2), k, res,
+                   bi + 2 * k, bd + 2 * k, st);
     for (int j = 0; j < k; ++j) {
-      ids[i * k + j] = bi[j] == kNone ? -1 : (int64_t)slot_id[bi[j]];
-      r2[i * k + j] = bd[j];
+      const uint32_t id = gld(bi + j);
+      gst(ids + i * k + j, id == kNone ? (int64_t)-1 : (int64_t)gld(slot_id + id));
+      gst(r2 + i * k + j, gld(bd + j));
     }
   }
-  if (x.st.overflow) report_error(err, kErrKnnStack, 0, 0, 0, 0, 0, 0);
-  flush_stats(stats, 0, 0, 0, 0, x.st.maxsp);
+  if (st.overflow) report_error(err, kErrKnnStack, 0, 0, 0, 0, 0, 0);
+  flush_stats(stats, 0, 0, 0, 0, st.maxsp);
 }
 
 }  // namespace vt
@@ -809,13 +807,15 @@ int vt_cloud_upload(vt_cloud *c) {
     c->tree_built = false;
     HIP_OK(hipGetDevice(&c->device));
     c->d_nodes = dev_alloc<Node>(t.nodes.size());
-    c->d_gen = dev_alloc<Gen>(c->n);
+    // kLeafMaxSize zero records of padding: leaf scans load whole leaves.
+    c->d_gen = dev_alloc<Gen>(c->n + kLeafMaxSize);
     c->d_fields = dev_alloc<double>(fslot.size());
     c->d_slot_id = dev_alloc<uint32_t>(c->n);
     c->d_err = dev_alloc<ErrRec>(1);
     c->d_stats = dev_alloc<Stats>(1);
     HIP_OK(hipMemcpy(c->d_nodes, t.nodes.data(), t.nodes.size() * sizeof(Node),
                      hipMemcpyHostToDevice));
+    gen.resize(c->n + kLeafMaxSize, Gen{0.0, 0.0, 0.0, 0.0});
     HIP_OK(hipMemcpy(c->d_gen, gen.data(), gen.size() * sizeof(Gen), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_fields, fslot.data(), std::max<size_t>(1, fslot.size()) * sizeof(double),
                      hipMemcpyHostToDevice));

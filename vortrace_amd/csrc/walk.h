@@ -5,6 +5,16 @@
 // only ever inserted right after the current one, so the points ahead of the
 // current one form a stack.  Here that stack keeps its top (the "next" point)
 // in registers and the rest in per-lane device memory, interleaved by lane.
+//
+// The walk is written as a state machine with ONE nearest-neighbour call site:
+// every step first advances the lane's state to its next query (endpoint,
+// dyadic probe, bisection probe, split-point classification, along-ray or
+// perpendicular perturbation), then runs the query, then consumes the result.
+// Lanes of a wave that are in different phases of their walks therefore share
+// the same traversal code, and only one copy of it (and of the periodic image
+// merge) is compiled.  Each step makes exactly the queries, in the order and
+// with the arithmetic, of the reference.
+//
 // Segments are handed to a sink as they close, in order, which fuses the
 // reductions of src/reduction.cpp into the walk.
 #pragma once
@@ -23,9 +33,9 @@ struct RayGeom {
 __device__ __forceinline__ void make_ray(const double *s, const double *e, RayGeom &r) {
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    r.a[j] = s[j];
-    r.b[j] = e[j];
-    r.d[j] = e[j] - s[j];
+    r.a[j] = gld(s + j);
+    r.b[j] = gld(e + j);
+    r.d[j] = r.b[j] - r.a[j];
   }
   const double L = sqrt(r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2]);
   r.d[0] /= L;
@@ -34,36 +44,21 @@ __device__ __forceinline__ void make_ray(const double *s, const double *e, RayGe
   r.len = L;
 }
 
-template <int kLdsDepth, int kBlock, bool kPer>
-struct WalkCtx {
-  static constexpr bool kPeriodic = kPer;
-  const DevCloud &C;
-  KStack<kLdsDepth, kBlock> st;
-  const Scratch &S;
-  int gtid;       // lane index among all persistent lanes
-  int lane;       // lane within the wave
-  int wave;       // global wave index (for the per-wave K_EXPAND buffer)
-  uint64_t nq;    // kNN queries issued
-  uint64_t nslow; // K_EXPAND searches
-  bool bad;       // a query found no generator (non-finite query point)
+// Lane identity and the per-lane scratch pointers a kNN stack needs.
+struct LaneInfo {
+  int tid;   // lane within the workgroup
+  int gtid;  // lane among all persistent lanes
+  int wave;  // global wave index (per-wave K_EXPAND buffer)
+  int lane;  // lane within the wave
+};
 
-  // Ids that index device arrays must be valid: a search that inserted
-  // nothing (NaN or overflowing coordinates) flags the ray and yields slot 0.
-  __device__ __forceinline__ uint32_t guard(uint32_t id) {
-    if (id == kNone) {
-      bad = true;
-      return 0;
-    }
-    return id;
-  }
+__device__ __forceinline__ KStack make_stack(const Scratch &S, const LaneInfo &L) {
+  return KStack{L.tid, S.knn_tag + L.gtid, S.knn_val + L.gtid, S.lanes, S.kspill, 0, 0, false, 0};
+}
 
-  __device__ __forceinline__ uint32_t query1(const double *q) {
-    uint32_t id[1];
-    double r2[1];
-    query_reg<1, kPer>(C, q[0], q[1], q[2], id, r2, st);
-    ++nq;
-    return guard(id[0]);
-  }
+struct WalkCounters {
+  uint64_t nq = 0;     // kNN queries
+  uint64_t nslow = 0;  // K_EXPAND searches
 };
 
 // findSplitPointDistance(pos1, pos2), src/ray.cpp:38-60.
@@ -80,18 +75,19 @@ __device__ __forceinline__ double split_analytic(const RayGeom &r, const double 
   return ndp / ndd;
 }
 
-// findSplitPointDistance(cloud, id1, id2, s_lo, s_hi), src/ray.cpp:64-127:
-// minimum-image generators in periodic mode, the analytic bisector, and a
-// 64-step bisection with 1-NN probes when that falls outside the segment.
-template <class X>
-__device__ int split_point(X &x, const RayGeom &r, uint32_t id1, uint32_t id2, double s_lo,
-                           double s_hi, double &out) {
-  const DevCloud &C = x.C;
-  const Gen g1 = C.gen[id1];
-  const Gen g2 = C.gen[id2];
-  double p1[3] = {g1.x, g1.y, g1.z};
-  double p2[3] = {g2.x, g2.y, g2.z};
-  if constexpr (X::kPeriodic) {
+// The analytic part of findSplitPointDistance(cloud, id1, id2, s_lo, s_hi)
+// (src/ray.cpp:64-94): minimum-image generators in periodic mode, then the
+// bisector.  Returns false when the reference falls back to bisection.
+template <bool kPer>
+__device__ __forceinline__ bool split_point(const DevCloud &C, const RayGeom &r, uint32_t id1,
+                                            uint32_t id2, double s_lo, double s_hi,
+                                            double &out) {
+  const double2 *g1 = reinterpret_cast<const double2 *>(C.gen + id1);
+  const double2 *g2 = reinterpret_cast<const double2 *>(C.gen + id2);
+  const double2 a1 = gld(g1), b1 = gld(g1 + 1), a2 = gld(g2), b2 = gld(g2 + 1);
+  double p1[3] = {a1.x, a1.y, b1.x};
+  double p2[3] = {a2.x, a2.y, b2.x};
+  if constexpr (kPer) {
     const double s_ref = 0.5 * (s_lo + s_hi);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -109,104 +105,28 @@ __device__ int split_point(X &x, const RayGeom &r, uint32_t id1, uint32_t id2, d
     }
   }
   const double s = split_analytic(r, p1, p2);
-  if (s < DBL_MAX * 0.5 && s >= s_lo - kTol && s <= s_hi + kTol) {
-    out = s;
-    return kErrNone;
-  }
-  double lo = s_lo, hi = s_hi;
-  for (int it = 0; it < 64; ++it) {
-    const double mid_s = 0.5 * (lo + hi);
-    double mid[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) mid[j] = r.a[j] + mid_s * r.d[j];
-    const uint32_t mc = x.query1(mid);
-    if (mc != id1 && mc != id2) {
-      out = mid_s;
-      return kErrNone;
-    }
-    if (mc == id1)
-      lo = mid_s;
-    else
-      hi = mid_s;
-    if (hi - lo < 1e-12) {
-      out = 0.5 * (lo + hi);
-      return kErrNone;
-    }
-  }
-  return kErrBisect;
+  out = s;
+  return s < DBL_MAX * 0.5 && s >= s_lo - kTol && s <= s_hi + kTol;
 }
 
-// Ray::perpPerturbToFindCell, src/ray.cpp:129-185.
-template <class X>
-__device__ __noinline__ int perp_perturb(X &x, const RayGeom &r, const double *pos,
-                                         uint32_t ex1, uint32_t ex2, uint32_t &out) {
-  const double s2 = x.C.inv_sqrt2;
-  const double w[3][3] = {{s2, s2, 0.0}, {0.0, s2, s2}, {s2, 0.0, s2}};
-  int cycle = 0;
-  for (int bound = 1; cycle < kMaxCycles; bound++)
-    for (int a = -bound; a <= bound && cycle < kMaxCycles; a++)
-      for (int b = -bound; b <= bound && cycle < kMaxCycles; b++)
-        for (int c = -bound; c <= bound && cycle < kMaxCycles; c++) {
-          if (a == 0 && b == 0 && c == 0) continue;
-          if (abs(a) < bound && abs(b) < bound && abs(c) < bound) continue;
-          double pd[3], cr[3];
-          for (int j = 0; j < 3; ++j) pd[j] = a * w[0][j] + b * w[1][j] + c * w[2][j];
-          cr[0] = r.d[1] * pd[2] - r.d[2] * pd[1];
-          cr[1] = r.d[2] * pd[0] - r.d[0] * pd[2];
-          cr[2] = r.d[0] * pd[1] - r.d[1] * pd[0];
-          const double mag = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
-          if (mag < 1e-15) {
-            cycle++;
-            continue;
-          }
-          for (int j = 0; j < 3; ++j) cr[j] /= mag;
-          double pp[3];
-          for (int j = 0; j < 3; ++j) pp[j] = pos[j] + kPertEps * cr[j];
-          const uint32_t cell = x.query1(pp);
-          if (cell != ex1 && cell != ex2) {
-            out = cell;
-            return kErrNone;
-          }
-          cycle++;
-        }
-  return kErrPerturb;
-}
-
-// The tryAlongRay lambda of src/ray.cpp:374-384.
-template <class X>
-__device__ uint32_t try_along_ray(X &x, const RayGeom &r, const double *pos, double sign,
-                                  uint32_t ct, uint32_t nt) {
-  double pp[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) pp[j] = pos[j] + sign * kPertEps * r.d[j];
-  uint32_t ids[2];
-  double r2[2];
-  query_reg<2, X::kPeriodic>(x.C, pp[0], pp[1], pp[2], ids, r2, x.st);
-  ++x.nq;
-  ids[0] = x.guard(ids[0]);
-  if (r2[0] + kTol < r2[1] && ids[0] != ct && ids[0] != nt) return ids[0];
-  return kNone;
-}
-
-// The K_EXPAND re-classification (src/ray.cpp:320-338) for the lanes of a
-// wave that need it, one lane at a time, in the wave's device buffer.
-// Sets has_c/has_n; returns an error code.
-template <class X>
-__device__ __noinline__ int classify_expand(X &x, const double *pos, uint32_t ct, uint32_t nt,
-                                            bool &has_c, bool &has_n) {
-  const DevCloud &C = x.C;
+// The K_EXPAND re-classification (src/ray.cpp:320-338), in the wave's device
+// buffer; the caller runs the lanes of a wave that need it one at a time.
+template <bool kPer>
+__device__ int classify_expand(const DevCloud &C, const Scratch &S, const LaneInfo &L,
+                               KStack &st, const double *pos, uint32_t ct, uint32_t nt,
+                               bool &has_c, bool &has_n) {
   const int kb = kExpandK < (int)C.n ? kExpandK : (int)C.n;
-  uint32_t *bi = x.S.big_id + (size_t)x.wave * 4 * kMaxQueryK;
-  double *bd = x.S.big_d + (size_t)x.wave * 4 * kMaxQueryK;
+  uint32_t *bi = S.big_id + (size_t)L.wave * 4 * kMaxQueryK;
+  double *bd = S.big_d + (size_t)L.wave * 4 * kMaxQueryK;
   MemRes res{bd, bi, kb, 0};
-  query_mem(C, pos[0], pos[1], pos[2], kb, res, bi + 2 * kMaxQueryK, bd + 2 * kMaxQueryK,
-            x.st);
-  ++x.nq;
-  ++x.nslow;
+  query_mem<kPer>(C, pos[0], pos[1], pos[2], kb, res, bi + 2 * kMaxQueryK, bd + 2 * kMaxQueryK,
+                  st);
+  const double d0 = gld(bd);
   for (int i = 0; i < kb; ++i) {
-    if (bd[i] - bd[0] >= kTol) break;
-    if (bi[i] == ct) has_c = true;
-    if (bi[i] == nt) has_n = true;
+    if (gld(bd + i) - d0 >= kTol) break;
+    const uint32_t id = gld(bi + i);
+    if (id == ct) has_c = true;
+    if (id == nt) has_n = true;
     if (i == kb - 1) {
       if (!has_c && !has_n) return kErrClassify;
       // Every entry is equidistant: which ties made the cut depends on the
@@ -217,175 +137,336 @@ __device__ __noinline__ int classify_expand(X &x, const double *pos, uint32_t ct
   return kErrNone;
 }
 
+// Walk states.  A state names the query the lane issues next.
+enum : int {
+  kStStart = 0,  // 1-NN of the ray start
+  kStEnd,        // 1-NN of the ray end
+  kStDyadic,     // periodic start: 1-NN at s = L * i / 2^level
+  kStSplit,      // (no query) next split point; becomes Classify or Bisect
+  kStBisect,     // 1-NN probe of the split-point bisection
+  kStClassify,   // 3-NN at the split point
+  kStAlongC,     // 2-NN at pos - eps * dir (case 1a, c side)
+  kStAlongN,     // 2-NN at pos + eps * dir (case 1a, n side)
+  kStPerpC,      // 1-NN of the perpendicular cycling, c side
+  kStPerpN,      // 1-NN of the perpendicular cycling, n side
+  kStDone,
+};
+
 // Walks one ray; emits its segments to `sink` in order.  Returns an error code
-// (kErrNone on success) and fills e0..e2 / i0,i1 with the values the host
-// formats into the reference's message.
-template <class X, class Sink>
-__device__ int walk_ray(X &x, const RayGeom &r, Sink &sink, int &max_depth, double *ev,
-                        uint32_t *ei) {
-  const DevCloud &C = x.C;
-  const Scratch &S = x.S;
-  x.bad = false;
-  const uint32_t c0 = x.query1(r.a);
-  const uint32_t c1 = x.query1(r.b);
-  if (x.bad) return kErrNonFinite;
-
-  // current point
-  uint32_t ct = c0;
-  double sc = 0.0;
-  // top of the pending stack ("next")
-  uint32_t nt = c1;
-  double sn = r.len;
+// (kErrNone on success) and fills ev / ei with the values the host formats
+// into the reference's message.
+template <bool kPer, class Sink>
+__device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, KStack &st,
+                        const RayGeom &r, Sink &sink, int &max_depth, WalkCounters &cnt,
+                        double *ev, uint32_t *ei) {
+  const size_t wbase = (size_t)L.gtid;
+  // current point, and the top of the pending stack ("next")
+  uint32_t ct = 0, nt = 0;
+  double sc = 0.0, sn = r.len;
   int wsp = 0;  // pending entries below the top, in device memory
-
-  uint32_t open_cell = c0;
+  uint32_t open_cell = 0;
   double entry_s = 0.0;
+  int iteration = 0;
+  int state = kStStart;
+  double s = 0.0;          // current split point
+  uint32_t c0 = 0;         // start cell
+  uint32_t side_cell = 0;  // case 1a: the c-side cell once resolved
+  // per-state registers: dyadic (level, i), bisection (it, lo, hi),
+  // perpendicular cycling (bound, a, b, c, cycle)
+  int ia = 0, ib = 0, ic = 0, id = 0, ie = 0;
+  double lo = 0.0, hi = 0.0;
 
-  auto push_top = [&](uint32_t id, double s) -> bool {
+  auto push_top = [&](uint32_t cell, double sv) -> bool {
     if (wsp >= S.wcap) return false;
-    S.walk_tag[(size_t)wsp * S.lanes + x.gtid] = nt;
-    S.walk_s[(size_t)wsp * S.lanes + x.gtid] = sn;
+    gst(S.walk_tag + (size_t)wsp * S.lanes + wbase, nt);
+    gst(S.walk_s + (size_t)wsp * S.lanes + wbase, sn);
     ++wsp;
     max_depth = wsp > max_depth ? wsp : max_depth;
-    nt = id;
-    sn = s;
+    nt = cell;
+    sn = sv;
     return true;
   };
-
-  if (c0 == c1) {
-    if constexpr (!X::kPeriodic) {
-      sink.seg(c0, 0.0, r.len);
-      return kErrNone;
-    }
-    // Dyadic probes s = L*i/2^level, src/ray.cpp:220-246.
-    bool found = false;
-    for (int level = 1; level <= 10 && !found; level++) {
-      const int n = 1 << level;
-      for (int i = 1; i < n && !found; i += 2) {
-        const double mid_s = r.len * i / n;
-        double mid[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) mid[j] = r.a[j] + mid_s * r.d[j];
-        const uint32_t mc = x.query1(mid);
-        if (mc != c0) {
-          if (!push_top(mc, mid_s)) return kErrWalkStack;
-          found = true;
-        }
-      }
-    }
-    if (x.bad) return kErrNonFinite;
-    if (!found) return kErrPeriodicStart;
-  }
-
-  int iteration = 0;
-  for (;;) {
-    if (x.bad) return kErrNonFinite;
-    if (++iteration > kMaxIter) return kErrMaxIter;
-    // Zero-width step (two points inserted at one split point).
-    if (sn - sc < 1e-14) {
-      ct = nt;
-      sc = sn;
-      open_cell = ct;
-      if (wsp == 0) break;
-      --wsp;
-      nt = S.walk_tag[(size_t)wsp * S.lanes + x.gtid];
-      sn = S.walk_s[(size_t)wsp * S.lanes + x.gtid];
-      continue;
-    }
-    double s;
-    int rc = split_point(x, r, ct, nt, sc, sn, s);
-    if (rc) {
-      ei[0] = ct;
-      ei[1] = nt;
-      return rc;
-    }
-    if (s < sc - kTol || s > sn + kTol) {
-      ev[0] = s;
+  // Advance the current point to the next one; false when none is left.
+  auto advance = [&]() -> bool {
+    ct = nt;
+    sc = sn;
+    if (wsp == 0) return false;
+    --wsp;
+    nt = gld(S.walk_tag + (size_t)wsp * S.lanes + wbase);
+    sn = gld(S.walk_s + (size_t)wsp * S.lanes + wbase);
+    return true;
+  };
+  // The split point was found (analytically or by bisection): bounds check,
+  // clamp (src/ray.cpp:280-291), then classify.
+  auto accept_split = [&](double sv) -> int {
+    if (sv < sc - kTol || sv > sn + kTol) {
+      ev[0] = sv;
       ev[1] = sc;
       ev[2] = sn;
       ei[0] = ct;
       ei[1] = nt;
       return kErrSplitOutside;
     }
-    {
-      const double lo = sc + 1e-15, hi = sn - 1e-15;
-      s = (s < lo) ? lo : ((hi < s) ? hi : s);  // std::clamp
+    const double l2 = sc + 1e-15, h2 = sn - 1e-15;
+    s = (sv < l2) ? l2 : ((h2 < sv) ? h2 : sv);  // std::clamp
+    state = kStClassify;
+    return kErrNone;
+  };
+  // Case 1a resolved on both sides: insert one or two points at s.
+  auto resolve_1a = [&](uint32_t scell, uint32_t ncell) -> int {
+    if (scell == ncell) {
+      if (!push_top(scell, s)) return kErrWalkStack;
+    } else {  // current -> scell -> ncell -> next, both at s
+      if (!push_top(ncell, s)) return kErrWalkStack;
+      if (!push_top(scell, s)) return kErrWalkStack;
+    }
+    state = kStSplit;
+    return kErrNone;
+  };
+  // Perpendicular cycling (src/ray.cpp:129-185): move to the next candidate
+  // direction that needs a query, counting degenerate ones as cycles.
+  // Returns false when MAX_PERTURBATION_CYCLES is reached.
+  auto perp_next = [&](double (&u)[3]) -> bool {
+    const double s2 = C.inv_sqrt2;
+    for (;;) {
+      if (ie >= kMaxCycles) return false;
+      // iterator (bound = ia, a = ib, b = ic, c = id) in the reference's loop order
+      if (++id > ia) {
+        id = -ia;
+        if (++ic > ia) {
+          ic = -ia;
+          if (++ib > ia) {
+            ++ia;
+            ib = ic = id = -ia;
+          }
+        }
+      }
+      const int a = ib, b = ic, c = id;
+      if (a == 0 && b == 0 && c == 0) continue;
+      if (abs(a) < ia && abs(b) < ia && abs(c) < ia) continue;
+      const double w[3][3] = {{s2, s2, 0.0}, {0.0, s2, s2}, {s2, 0.0, s2}};
+      double pd[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) pd[j] = a * w[0][j] + b * w[1][j] + c * w[2][j];
+      u[0] = r.d[1] * pd[2] - r.d[2] * pd[1];
+      u[1] = r.d[2] * pd[0] - r.d[0] * pd[2];
+      u[2] = r.d[0] * pd[1] - r.d[1] * pd[0];
+      const double mag = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      if (mag < 1e-15) {
+        ++ie;
+        continue;
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) u[j] /= mag;
+      return true;
+    }
+  };
+  auto perp_start = [&](int st_next) {
+    ia = 1;   // bound
+    ib = -1;  // a
+    ic = -1;  // b
+    id = -2;  // c (the first perp_next moves it to -1)
+    ie = 0;   // cycle
+    state = st_next;
+  };
+
+  for (;;) {
+    // ---- 1. advance to the next query ----
+    if (state == kStSplit) {
+      for (;;) {
+        if (++iteration > kMaxIter) return kErrMaxIter;
+        if (sn - sc < 1e-14) {  // zero-width step
+          const bool more = advance();
+          open_cell = ct;
+          if (!more) {
+            state = kStDone;
+            break;
+          }
+          continue;
+        }
+        double sv;
+        if (split_point<kPer>(C, r, ct, nt, sc, sn, sv)) {
+          const int rc = accept_split(sv);
+          if (rc) return rc;
+        } else {
+          lo = sc;
+          hi = sn;
+          ia = 0;
+          state = kStBisect;
+        }
+        break;
+      }
+      if (state == kStDone) break;
     }
     double pos[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) pos[j] = r.a[j] + s * r.d[j];
+    double q[3];
+    int k = 1;
+    if (state == kStStart) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) q[j] = r.a[j];
+    } else if (state == kStEnd) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) q[j] = r.b[j];
+    } else if (state == kStDyadic) {
+      const double mid_s = r.len * ib / (1 << ia);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) q[j] = r.a[j] + mid_s * r.d[j];
+    } else if (state == kStBisect) {
+      const double mid_s = 0.5 * (lo + hi);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) q[j] = r.a[j] + mid_s * r.d[j];
+    } else if (state == kStClassify) {
+#pragma unroll
+      for (int j = 0; j < 3; ++j) q[j] = pos[j];
+      k = 3;
+    } else if (state == kStAlongC || state == kStAlongN) {
+      const double sign = state == kStAlongC ? -1.0 : 1.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) q[j] = pos[j] + sign * kPertEps * r.d[j];
+      k = 2;
+    } else {  // kStPerpC / kStPerpN: around pos -/+ eps * dir
+      double u[3];
+      if (!perp_next(u)) return kErrPerturb;
+      const bool c_side = state == kStPerpC;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double base = c_side ? pos[j] - kPertEps * r.d[j] : pos[j] + kPertEps * r.d[j];
+        q[j] = base + kPertEps * u[j];
+      }
+    }
 
+    // ---- 2. the query ----
     uint32_t ids[3];
     double r2[3];
-    query_reg<3, X::kPeriodic>(C, pos[0], pos[1], pos[2], ids, r2, x.st);
-    ++x.nq;
-    ids[0] = x.guard(ids[0]);
-    if (x.bad) return kErrNonFinite;
-    bool has_c = false, has_n = false, three_eq = true;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      if (three_eq) {
-        if (r2[i] - r2[0] >= kTol) {
-          three_eq = false;
-        } else {
-          if (ids[i] == ct) has_c = true;
-          if (ids[i] == nt) has_n = true;
+    query_rt<kPer>(C, q[0], q[1], q[2], k, ids, r2, st);
+    ++cnt.nq;
+    // Ids index device arrays: a search that inserted nothing (NaN or
+    // overflowing coordinates) fails the ray instead.
+    if (ids[0] == kNone) return kErrNonFinite;
+
+    // ---- 3. consume the result ----
+    if (state == kStStart) {
+      c0 = ids[0];
+      state = kStEnd;
+    } else if (state == kStEnd) {
+      const uint32_t c1 = ids[0];
+      ct = c0;
+      nt = c1;
+      open_cell = c0;
+      if (c0 != c1) {
+        state = kStSplit;
+      } else if constexpr (!kPer) {
+        sink.seg(c0, 0.0, r.len);
+        return kErrNone;
+      } else {
+        ia = 1;  // level
+        ib = 1;  // i
+        state = kStDyadic;
+      }
+    } else if (state == kStDyadic) {
+      if (ids[0] != c0) {
+        const double mid_s = r.len * ib / (1 << ia);
+        if (!push_top(ids[0], mid_s)) return kErrWalkStack;
+        state = kStSplit;
+      } else {
+        ib += 2;
+        if (ib >= (1 << ia)) {
+          ++ia;
+          ib = 1;
+          if (ia > 10) return kErrPeriodicStart;
         }
       }
-    }
-    if (three_eq && !has_c && !has_n) {
-      // Serialise the lanes of this wave that need the wide search.
-      rc = kErrNone;
-      uint64_t want = __ballot(1);
-      const int me = x.lane;
-      while (want) {
-        const int l = __ffsll((unsigned long long)want) - 1;
-        if (l == me) rc = classify_expand(x, pos, ct, nt, has_c, has_n);
-        want &= want - 1;
+    } else if (state == kStBisect) {
+      const double mid_s = 0.5 * (lo + hi);
+      const uint32_t mc = ids[0];
+      int rc = kErrNone;
+      if (mc != ct && mc != nt) {
+        rc = accept_split(mid_s);
+      } else {
+        if (mc == ct)
+          lo = mid_s;
+        else
+          hi = mid_s;
+        if (hi - lo < 1e-12) {
+          rc = accept_split(0.5 * (lo + hi));
+        } else if (++ia >= 64) {
+          ei[0] = ct;
+          ei[1] = nt;
+          return kErrBisect;
+        }
       }
       if (rc) return rc;
-    }
-
-    if (has_c || has_n) {
-      // Case 2: the c/n boundary.
-      sink.seg(open_cell, entry_s, s);
-      open_cell = nt;
-      entry_s = s;
-      ct = nt;
-      sc = sn;
-      if (wsp == 0) break;
-      --wsp;
-      nt = S.walk_tag[(size_t)wsp * S.lanes + x.gtid];
-      sn = S.walk_s[(size_t)wsp * S.lanes + x.gtid];
-    } else if (r2[0] + kTol < r2[1]) {
-      // Case 1: one cell strictly closer; insert it.
-      if (!push_top(ids[0], s)) return kErrWalkStack;
-    } else {
-      // Case 1a: resolve each side by perturbation along, then across, the ray.
-      uint32_t scell = try_along_ray(x, r, pos, -1.0, ct, nt);
-      if (scell == kNone) {
-        double pc[3];
+    } else if (state == kStClassify) {
+      bool has_c = false, has_n = false, three_eq = true;
 #pragma unroll
-        for (int j = 0; j < 3; ++j) pc[j] = pos[j] - kPertEps * r.d[j];
-        rc = perp_perturb(x, r, pc, ct, nt, scell);
+      for (int i = 0; i < 3; ++i) {
+        if (three_eq) {
+          if (r2[i] - r2[0] >= kTol) {
+            three_eq = false;
+          } else {
+            if (ids[i] == ct) has_c = true;
+            if (ids[i] == nt) has_n = true;
+          }
+        }
+      }
+      if (three_eq && !has_c && !has_n) {
+        int rc = kErrNone;
+        uint64_t want = __ballot(1);
+        while (want) {
+          const int l = __ffsll((unsigned long long)want) - 1;
+          if (l == L.lane) {
+            rc = classify_expand<kPer>(C, S, L, st, pos, ct, nt, has_c, has_n);
+            ++cnt.nq;
+            ++cnt.nslow;
+          }
+          want &= want - 1;
+        }
         if (rc) return rc;
       }
-      if (x.bad) return kErrNonFinite;
-      uint32_t ncell = try_along_ray(x, r, pos, 1.0, ct, nt);
-      if (ncell == kNone) {
-        double pn[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) pn[j] = pos[j] + kPertEps * r.d[j];
-        rc = perp_perturb(x, r, pn, ct, nt, ncell);
-        if (rc) return rc;
-      }
-      if (x.bad) return kErrNonFinite;
-      if (scell == ncell) {
-        if (!push_top(scell, s)) return kErrWalkStack;
+      if (has_c || has_n) {
+        // Case 2: the c/n boundary.
+        sink.seg(open_cell, entry_s, s);
+        open_cell = nt;
+        entry_s = s;
+        if (!advance()) break;
+        state = kStSplit;
+      } else if (r2[0] + kTol < r2[1]) {
+        // Case 1: one cell strictly closer; insert it.
+        if (!push_top(ids[0], s)) return kErrWalkStack;
+        state = kStSplit;
       } else {
-        // current -> scell -> ncell -> next, both at s
-        if (!push_top(ncell, s)) return kErrWalkStack;
-        if (!push_top(scell, s)) return kErrWalkStack;
+        // Case 1a: resolve each side by perturbation along, then across, the ray.
+        state = kStAlongC;
+      }
+    } else if (state == kStAlongC || state == kStAlongN) {
+      const bool ok = r2[0] + kTol < r2[1] && ids[0] != ct && ids[0] != nt;
+      if (state == kStAlongC) {
+        if (ok) {
+          side_cell = ids[0];
+          state = kStAlongN;
+        } else {
+          perp_start(kStPerpC);
+        }
+      } else if (ok) {
+        const int rc = resolve_1a(side_cell, ids[0]);
+        if (rc) return rc;
+      } else {
+        perp_start(kStPerpN);
+      }
+    } else {  // kStPerpC / kStPerpN
+      const uint32_t cell = ids[0];
+      if (cell != ct && cell != nt) {
+        if (state == kStPerpC) {
+          side_cell = cell;
+          state = kStAlongN;
+        } else {
+          const int rc = resolve_1a(side_cell, cell);
+          if (rc) return rc;
+        }
+      } else {
+        ++ie;  // cycle
       }
     }
   }
