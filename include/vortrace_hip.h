@@ -119,9 +119,11 @@ int vt_project_segments(const vt_cloud *cloud, const double *d_start, const doub
                         void *stream);
 
 /* Per-call statistics of the last vt_project* call on this thread: total
- * segments (cell crossings), kNN queries and the deepest walk stack. */
+ * segments (cell crossings), kNN queries run, the deepest walk and kNN
+ * stacks, K_EXPAND searches, rays re-run by the K_EXPAND-capable kernel, and
+ * the kernels' time. */
 typedef struct {
-  int64_t segments, queries, max_walk_depth, max_knn_depth, slow_queries;
+  int64_t segments, queries, max_walk_depth, max_knn_depth, slow_queries, deferred_rays;
   float kernel_ms;
 } vt_call_stats;
 int vt_last_call_stats(vt_call_stats *out);

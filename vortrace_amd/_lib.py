@@ -24,7 +24,8 @@ class TreeInfo(C.Structure):
 class CallStats(C.Structure):
     _fields_ = [("segments", C.c_int64), ("queries", C.c_int64),
                 ("max_walk_depth", C.c_int64), ("max_knn_depth", C.c_int64),
-                ("slow_queries", C.c_int64), ("kernel_ms", C.c_float)]
+                ("slow_queries", C.c_int64), ("deferred_rays", C.c_int64),
+                ("kernel_ms", C.c_float)]
 
 
 WARN_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)

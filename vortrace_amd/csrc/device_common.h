@@ -26,7 +26,7 @@ constexpr double kPertEps = 2 * kTol;
 constexpr int kMaxCycles = 1000;
 constexpr int kExpandK = 100;  // K_EXPAND, src/ray.cpp:321
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr uint32_t kUndo = 0x80000000u;
+constexpr uint32_t kUndo = kRefLeaf | kRefUndo;  // stack tag of an undo entry (| dim)
 
 // Error codes raised inside kernels; the host maps them to the reference's
 // exception texts.
@@ -43,6 +43,9 @@ enum : int {
   kErrTies = 9,            // >= K_EXPAND equidistant generators at one point
   kErrNonFinite = 10,      // a query found no generator (non-finite point)
   kErrFillOverflow = 11,   // fill pass produced more segments than counted
+  // Not an error: the ray needs the K_EXPAND classification, which the fast
+  // kernels leave to a second launch over the deferred rays.
+  kDeferRay = 64,
 };
 
 struct ErrRec {
@@ -61,12 +64,13 @@ struct Stats {
   unsigned int max_knn_depth;
   unsigned long long slow_queries;
   unsigned long long ray_counter;  // work queue head
+  unsigned long long deferred;     // rays appended to the deferred list
 };
 
 // Everything a kernel needs to know about a built cloud (passed by value).
 struct DevCloud {
-  const Node *nodes;
-  const Gen *gen;        // slot order
+  const DNode *nodes;    // inner nodes, pre-order
+  const Gen *gen;        // slot order, followed by kLeafMaxSize padding records
   const double *fields;  // slot order, nf per slot
   double rb_lo[3], rb_hi[3];  // nanoflann root_bbox
   double sb[6];               // PointCloud::subbox
@@ -74,19 +78,22 @@ struct DevCloud {
   double inv_sqrt2;           // 1.0 / std::sqrt(2.0), computed on the host
   uint32_t n, nf;
   int periodic;
-  int pad;
+  uint32_t root;  // reference of the root (a leaf when n <= kLeafMaxSize)
 };
 
 // Per-lane scratch that does not fit in registers/LDS (device memory).
 struct Scratch {
   uint32_t *knn_tag;   // [kspill][T] kNN stack entries beyond the LDS part
   double *knn_val;
+  float *knn_pb;
   uint32_t *walk_tag;  // [wcap][T] pending ray points below the top
   double *walk_s;
   uint32_t *big_id;    // [T][3 * kMaxQueryK] k > 3 result / merge buffers
   double *big_d;
   int kspill, wcap;
   int lanes;           // T
+  int walk_batch;      // ray engine: lanes of a wave that run their walk steps together
+  int leaf_batch;      // ray engine: lanes of a wave that run their leaf steps together
   int pad;
 };
 
@@ -110,19 +117,40 @@ __device__ __forceinline__ double2 gld(const double2 *p) {
   return make_double2(v.x, v.y);
 }
 
-__device__ __forceinline__ Node load_node(const Node *nodes, uint32_t i) {
-  const double2 *p = reinterpret_cast<const double2 *>(nodes + i);
-  const double2 v0 = gld(p);
-  const double2 v1 = gld(p + 1);
-  Node n;
+// a, b or c for f = 0, 1 or 2 (by value).
+__device__ __forceinline__ double sel3(double a, double b, double c, uint32_t f) {
+  return f == 0 ? a : (f == 1 ? b : c);
+}
+
+// A device node, decoded.
+struct DNodeV {
+  double lo, hi, pb;
+  uint32_t c1, c2;  // child references (dimension bits removed)
+  uint32_t dim;
+};
+
+__device__ __forceinline__ DNodeV decode_node(const double2 v0, const double2 v1) {
+  DNodeV n;
   n.lo = v0.x;
   n.hi = v0.y;
-  uint64_t ab = __double_as_longlong(v1.x);
-  n.a = static_cast<uint32_t>(ab);
-  n.b = static_cast<uint32_t>(ab >> 32);
-  n.depth = 0;
-  n.pad = 0;
+  const uint64_t cc = (uint64_t)__double_as_longlong(v1.x);
+  const uint32_t c1 = (uint32_t)cc, c2 = (uint32_t)(cc >> 32);
+  n.dim = ((c1 >> 30) & 1u) | ((c2 >> 29) & 2u);
+  n.c1 = c1 & ~kDimBit;
+  n.c2 = c2 & ~kDimBit;
+  n.pb = v1.y;
   return n;
+}
+
+__device__ __forceinline__ DNodeV load_node(const DNode *nodes, uint32_t i) {
+  const double2 *p = reinterpret_cast<const double2 *>(nodes + i);
+  return decode_node(gld(p), gld(p + 1));
+}
+
+// Stack tag of a child reference taken as a far child at a node cutting
+// along `dim`.
+__device__ __forceinline__ uint32_t far_tag(uint32_t ref, uint32_t dim) {
+  return (ref & kRefLeaf) ? ref : (ref | (dim << 29));
 }
 
 __device__ __forceinline__ void report_error(ErrRec *err, int code, int64_t ray, double a,

@@ -1,0 +1,458 @@
+// The ray engine: every lane of a persistent wave owns one ray at a time and
+// advances it one small step per loop iteration.
+//
+// A ray alternates between its walk (walk.h: consume a query result, emit
+// segments, prepare the next query) and one kNN search at a time
+// (PointCloud::queryTreeK, src/pointcloud.cpp:308-368, over nanoflann 1.3.2's
+// searchLevel, include/nanoflann.hpp:1346-1408).  A search is unrolled into
+// node steps, which read one 32-byte inner node, and leaf steps, which read a
+// whole leaf's generators at once and then pop the stack.  Lanes of a wave run
+// their own searches at their own depths; a lane whose search ends does not
+// wait for the others.
+//
+// Leaf steps and walk steps are long compared with node steps.  The wave runs
+// node steps every iteration of a tight inner loop and leaf steps only once
+// S.leaf_batch of its lanes are at a leaf (or none is at a node); it leaves
+// the loop once S.walk_batch lanes wait for a walk step (or none is
+// searching), and then every waiting lane runs its walk step.
+//
+// Each search visits nodes and leaves, inserts points and prunes entries in
+// exactly nanoflann's order with nanoflann's arithmetic (see knn.h for the two
+// exact transformations of its recursion used here: prune-at-push and the
+// first-descent ring), so ids, r^2 bits and tie order are the reference's.
+#pragma once
+#include "walk.h"
+
+namespace vt {
+
+// Search modes.
+enum : int {
+  kTmNode = 0,   // visit inner node `cur`
+  kTmEnter = 1,  // visit inner node `cur`, a far child taken off the stack
+  kTmLeaf = 2,   // scan the leaf whose reference is `cur`
+  kTmPop = 3,    // take the next entry off the stack
+  kTmDone = 4,   // search finished
+};
+
+// Generators a leaf step loads at once.
+constexpr int kLeafChunk = 5;
+
+// One queryTreeK for k in {1, 2, 3}: the plain search, then in periodic mode
+// the image searches, each run as a sequence of node and leaf steps.
+template <bool kPer>
+struct Search {
+  double q[3];  // point of the running search (an image in periodic mode)
+  double dd[3];
+  double mind;
+  RegRes3 rs;
+  uint32_t cur;  // inner node index, or leaf reference
+  int mode;
+  int depth;            // >= 0 on the first descent (LDS ring depth), else -1
+  double lost_min;      // smallest box distance that fell out of the ring
+  uint32_t first_leaf;  // first descent's leaf; a re-descent stops there
+  double pv;            // popped entry: box distance of the far child
+  uint32_t pdim;        // popped entry: the parent's cut dimension
+  float ptop;           // lower bound of the box distances on the stack
+  // periodic images (src/pointcloud.cpp:327-365)
+  double qb[3];  // the query point itself
+  uint32_t mi[3];  // merged result so far (m <= k entries between images)
+  double md[3];
+  int m;
+  int img;  // -1: the plain search; else the image (ix+1)*9 + (iy+1)*3 + (iz+1)
+
+  // computeInitialDistances against root_bbox, then the first descent.
+  __device__ __forceinline__ void begin_search(const DevCloud &C, KStack &st, double x,
+                                               double y, double z) {
+    q[0] = x;
+    q[1] = y;
+    q[2] = z;
+    rs.init();
+    mind = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      dd[i] = 0.0;
+      if (q[i] < C.rb_lo[i]) {
+        dd[i] = (q[i] - C.rb_lo[i]) * (q[i] - C.rb_lo[i]);
+        mind += dd[i];
+      }
+      if (q[i] > C.rb_hi[i]) {
+        dd[i] = (q[i] - C.rb_hi[i]) * (q[i] - C.rb_hi[i]);
+        mind += dd[i];
+      }
+    }
+    st.sp = 0;
+    st.base = 0;
+    ptop = INFINITY;
+    cur = C.root;
+    mode = (cur & kRefLeaf) ? kTmLeaf : kTmNode;
+    depth = 0;
+    lost_min = DBL_MAX;
+    first_leaf = cur;
+  }
+
+  __device__ __forceinline__ void start(const DevCloud &C, KStack &st, const double (&x)[3],
+                                        int k) {
+    qb[0] = x[0];
+    qb[1] = x[1];
+    qb[2] = x[2];
+    rs.k = k;
+    img = -1;
+    begin_search(C, st, x[0], x[1], x[2]);
+  }
+
+  // Component f of q / dd.  Values are selected, never addresses: a select
+  // of element addresses would keep the struct out of registers.
+  __device__ __forceinline__ double qd(uint32_t f) const { return sel3(q[0], q[1], q[2], f); }
+  __device__ __forceinline__ double ddd(uint32_t f) const {
+    return sel3(dd[0], dd[1], dd[2], f);
+  }
+  __device__ __forceinline__ void set_dd(uint32_t f, double v) {
+    dd[0] = f == 0 ? v : dd[0];
+    dd[1] = f == 1 ? v : dd[1];
+    dd[2] = f == 2 ? v : dd[2];
+  }
+
+  // One inner node (modes kTmNode / kTmEnter): choose the child to visit
+  // first and push the other.
+  __device__ __forceinline__ void node_step(const DevCloud &C, KStack &st) {
+    const DNodeV nd = load_node(C.nodes, cur);
+    if (mode == kTmEnter) {
+      // Entering the far child of a popped entry: dists[idx] = cut_dist,
+      // mindistsq from the entry; the old dists[idx] is restored on return.
+      const double c = qd(pdim) - nd.pb;
+      st.push(kUndo | pdim, ddd(pdim), ptop);
+      set_dd(pdim, c * c);
+      mind = pv;
+    }
+    const uint32_t f = nd.dim;
+    double cut;
+    const bool low_first = choose(nd, qd(f), cut);
+    const double m2 = mind + cut - ddd(f);
+    const uint32_t tag = far_tag(low_first ? nd.c2 : nd.c1, f);
+    if (depth >= 0) {
+      // First descent (worst distance still infinite): every far child goes
+      // into the LDS ring at slot depth mod kLdsDepth.
+      const int sl = (depth & (kLdsDepth - 1)) * kBlock + st.tid;
+      if (depth >= kLdsDepth) {
+        const double old = lds_stack_val[sl];
+        lost_min = old < lost_min ? old : lost_min;
+      }
+      lds_stack_tag[sl] = tag;
+      lds_stack_val[sl] = m2;
+      lds_stack_pb[sl] = ptop;
+      ptop = fminf(ptop, bound_below(m2));
+      ++depth;
+    } else if (m2 <= rs.worst()) {
+      st.push(tag, m2, ptop);
+      ptop = fminf(ptop, bound_below(m2));
+    }
+    const uint32_t near = low_first ? nd.c1 : nd.c2;
+    cur = near;
+    mode = kTmNode;
+    if (near & kRefLeaf) {
+      mode = kTmLeaf;
+      if (depth < 0 && near == first_leaf) mode = kTmPop;  // re-descent is back at the first leaf
+      if (depth >= 0) first_leaf = near;
+    }
+  }
+
+  // Takes entries off the stack until one is visited (mode kTmEnter or
+  // kTmLeaf) or nothing left can be (kTmDone).
+  __device__ __forceinline__ void pop(KStack &st) {
+    const double worst = rs.worst();
+    for (;;) {
+      if (st.sp == 0) break;
+      uint32_t tag;
+      double v;
+      float pb;
+      st.pop(tag, v, pb);
+      ptop = pb;
+      if (tag == kSkip) continue;
+      if ((tag & kUndo) == kUndo) {
+        set_dd(tag & 3u, v);
+      } else if (v <= worst) {  // mindistsq * epsError <= worstDist()
+        if (tag & kRefLeaf) {
+          cur = tag;
+          mode = kTmLeaf;
+        } else {
+          cur = tag & kIdxMask;
+          pdim = (tag >> 29) & 3u;
+          pv = v;
+          mode = kTmEnter;
+        }
+        return;
+      }
+      // Nothing below can be visited any more: nanoflann prunes all of it.
+      if ((double)pb > worst) break;
+    }
+    st.sp = 0;
+    mode = kTmDone;
+  }
+
+  // One leaf (mode kTmLeaf) then the stack (kTmPop).
+  __device__ __forceinline__ void leaf_step(const DevCloud &C, KStack &st) {
+    if (mode == kTmLeaf) {
+      // searchLevel's leaf loop (include/nanoflann.hpp:1349-1366).  Testing
+      // against the current worst distance instead of the one read at the
+      // leaf's start inserts the same points: addPoint ignores a distance
+      // that is not below the worst once the set is full.  Generators are
+      // loaded kLeafChunk at a time; loads past the leaf's end stay inside
+      // the padding records after the last slot.
+      const uint32_t s0 = cur & kSlotMask;
+      const uint32_t end = (uint32_t)gld(&C.gen[s0].end);
+      uint32_t t0 = s0;
+      do {
+        double2 xy[kLeafChunk];
+        double z[kLeafChunk];
+#pragma unroll
+        for (int j = 0; j < kLeafChunk; ++j) {
+          xy[j] = gld(reinterpret_cast<const double2 *>(C.gen + t0 + j));
+          z[j] = gld(&C.gen[t0 + j].z);
+        }
+#pragma unroll
+        for (int j = 0; j < kLeafChunk; ++j) {
+          const double e0 = q[0] - xy[j].x;
+          const double e1 = q[1] - xy[j].y;
+          const double e2 = q[2] - z[j];
+          double dist = e0 * e0;
+          dist = dist + e1 * e1;
+          dist = dist + e2 * e2;
+          if (t0 + j < end && dist < rs.worst()) rs.add(dist, t0 + j);
+        }
+        t0 += kLeafChunk;
+      } while (t0 < end);
+      mode = kTmPop;
+      if (depth >= 0) {
+        // End of the first descent: the ring holds the far children of the
+        // deepest kLdsDepth levels.
+        const double w0 = rs.worst();
+        if (depth <= kLdsDepth || !(lost_min <= w0)) {
+          st.sp = depth < kLdsDepth ? depth : kLdsDepth;
+          st.base = depth > kLdsDepth ? (depth & (kLdsDepth - 1)) : 0;
+          st.maxsp = st.sp > st.maxsp ? st.sp : st.maxsp;
+        } else {
+          // An entry that fell out of the ring can still be visited: walk
+          // the same path again, pushing the far children that can.
+          st.sp = 0;
+          st.base = 0;
+          ptop = INFINITY;
+          cur = C.root;
+          mode = kTmNode;
+        }
+        depth = -1;
+      }
+    }
+    if (mode == kTmPop) pop(st);
+  }
+
+  // Called when a search has finished.  Returns true with the query's result
+  // in ids / r2 (entries >= k are kNone / DBL_MAX), or false after starting
+  // the next image search.
+  __device__ __forceinline__ bool finish(const DevCloud &C, KStack &st, uint32_t (&ids)[3], double (&r2)[3]) {
+    const int k = rs.k;
+    if constexpr (kPer) {
+      if (img < 0) {
+        ImageBox ib;
+        ib.init(C, qb);
+        if (ib.any_within(rs.worst() + kTol)) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            mi[j] = rs.id[j];
+            md[j] = rs.d[j];
+          }
+          m = k;
+          img = 0;
+        }
+      } else {
+        // Merge: de-duplicate by id keeping the smaller r^2, stable sort by
+        // r^2 (std::sort on at most 16 elements is libstdc++'s insertion
+        // sort), truncate to k.  Between images at most k <= 3 entries
+        // survive; up to 2k are live during the merge.
+        uint32_t xi[6];
+        double xd[6];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          xi[j] = mi[j];
+          xd[j] = md[j];
+          xi[j + 3] = kNone;
+          xd[j + 3] = DBL_MAX;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          if (i < k) {
+            bool found = false;
+#pragma unroll
+            for (int j = 0; j < 6; ++j) {
+              if (!found && j < m && xi[j] == rs.id[i]) {
+                xd[j] = (rs.d[i] < xd[j]) ? rs.d[i] : xd[j];
+                found = true;
+              }
+            }
+            if (!found) {
+#pragma unroll
+              for (int j = 0; j < 6; ++j)
+                if (j == m) {
+                  xi[j] = rs.id[i];
+                  xd[j] = rs.d[i];
+                }
+              ++m;
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 1; i < 6; ++i) {
+#pragma unroll
+          for (int j = i; j > 0; --j) {
+            if (i < m && xd[j] < xd[j - 1]) {
+              const double td = xd[j];
+              xd[j] = xd[j - 1];
+              xd[j - 1] = td;
+              const uint32_t ti = xi[j];
+              xi[j] = xi[j - 1];
+              xi[j - 1] = ti;
+            }
+          }
+        }
+        if (m > k) m = k;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          mi[j] = xi[j];
+          md[j] = xd[j];
+        }
+        ++img;
+      }
+      if (img >= 0) {
+        for (; img < 27; ++img) {
+          if (img == 13) continue;
+          const int ix = img / 9 - 1, iy = (img / 3) % 3 - 1, iz = img % 3 - 1;
+          const double sh[3] = {qb[0] + ix * C.box[0], qb[1] + iy * C.box[1],
+                                qb[2] + iz * C.box[2]};
+          const double wk = sel3(md[0], md[1], md[2], (uint32_t)(k - 1));
+          if (min_dist_sq_to_box(C, sh) > wk + kTol) continue;
+          begin_search(C, st, sh[0], sh[1], sh[2]);
+          return false;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          ids[j] = j < k ? mi[j] : kNone;
+          r2[j] = j < k ? md[j] : DBL_MAX;
+        }
+        return true;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      ids[j] = j < k ? rs.id[j] : kNone;
+      r2[j] = j < k ? rs.d[j] : DBL_MAX;
+    }
+    return true;
+  }
+};
+
+// Lane phases.
+enum : int {
+  kPhFetch = 0,  // needs a ray
+  kPhWalk = 1,   // a search has finished: walk step due
+  kPhSearch = 2,
+  kPhDone = 3,   // no rays left
+};
+
+struct EngineCounters {
+  uint64_t nseg = 0, nq = 0, nslow = 0;
+  int maxw = 0;
+};
+
+// Runs rays through the walk, handing segments to the sinks IO provides:
+// IO::Sink, io.begin(ray, sink), io.end(ray, sink) -> error code (kErrNone or
+// a failure of the ray's output).  The rays are [0, nrays), or ray_list[0,
+// nrays) when ray_list is given.  Without kExpand, a ray that needs the
+// K_EXPAND classification is abandoned and appended to `defer` (its outputs
+// are rewritten by the launch that runs the deferred rays with kExpand).
+template <bool kPer, bool kExpand, class IO>
+__device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
+                                           const double *start, const double *end,
+                                           int64_t nrays, const uint32_t *ray_list,
+                                           uint32_t *defer, ErrRec *err, Stats *stats, IO &io,
+                                           KStack &st, EngineCounters &cnt) {
+  const int gtid = blockIdx.x * kBlock + threadIdx.x;
+  const size_t wbase = (size_t)gtid;
+  Walker<kPer> W;
+  Search<kPer> Q;
+  typename IO::Sink sk;
+  int ph = kPhFetch;
+  int64_t ray = 0;
+  const int batch = S.walk_batch;
+  const int leaf_batch = S.leaf_batch;
+  for (;;) {
+    {  // walk steps of every waiting lane
+      WalkErr e{{0.0, 0.0, 0.0}, {0u, 0u}};
+      int rc = kErrNone;
+      bool query = false;
+      double q[3];
+      int k = 1;
+      if (ph == kPhWalk) {
+        uint32_t ids[3];
+        double r2[3];
+        if (Q.finish(C, st, ids, r2)) {
+          ++cnt.nq;
+          rc = W.template consume<kExpand>(C, S, wbase, (int)threadIdx.x, gtid, ids, r2, sk, cnt.maxw, cnt.nslow, e);
+          if (!rc && W.state < kStFinal) rc = W.prepare(C, S, wbase, end + 3 * ray, q, k, e);
+          if (!rc && W.state == kStFinal) {
+            sk.seg(W.open_cell, W.entry_s, W.sc);
+            W.state = kStDone;
+          }
+          if (!rc && W.state == kStDone && st.overflow) rc = kErrKnnStack;
+          if (!rc && W.state == kStDone) rc = io.end(ray, sk);
+          if (!kExpand && rc == kDeferRay) {
+            const unsigned long long slot = atomicAdd(&stats->deferred, 1ULL);
+            gst(defer + slot, (uint32_t)ray);
+            ph = kPhFetch;
+          } else if (rc || W.state == kStDone) {
+            if (rc) report_error(err, rc, ray, e.v[0], e.v[1], e.v[2], e.id[0], e.id[1]);
+            cnt.nseg += sk.n;
+            ph = kPhFetch;
+          } else {
+            query = true;
+          }
+        } else {
+          ph = kPhSearch;  // the next image search has started
+        }
+      }
+      if (ph == kPhFetch) {
+        ray = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
+        if (ray >= nrays ||
+            __hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ph = kPhDone;
+        } else {
+          if (ray_list) ray = (int64_t)gld(ray_list + ray);
+          W.begin(start + 3 * ray, end + 3 * ray);
+          io.begin(ray, sk);
+          W.prepare(C, S, wbase, end + 3 * ray, q, k, e);  // kStStart: the ray's start
+          query = true;
+        }
+      }
+      if (query) {
+        Q.start(C, st, q, k);
+        ph = kPhSearch;
+      }
+    }
+    if (!__ballot(ph != kPhDone)) break;
+    // Search steps only, until `batch` lanes wait for a walk step or no lane
+    // is searching.  Node steps run every iteration; leaf steps once
+    // `leaf_batch` lanes (or every searching lane) are at a leaf.
+    for (;;) {
+      const bool at_leaf = ph == kPhSearch && Q.mode >= kTmLeaf;
+      const uint64_t nodes = __ballot(ph == kPhSearch && Q.mode <= kTmEnter);
+      const uint64_t leaves = __ballot(at_leaf);
+      if (!(nodes | leaves)) break;
+      if (__popcll(__ballot(ph == kPhWalk || ph == kPhFetch)) >= batch) break;
+      if (leaves && (!nodes || __popcll(leaves) >= leaf_batch) && at_leaf) {
+        Q.leaf_step(C, st);
+        if (Q.mode == kTmDone) ph = kPhWalk;
+      }
+      if (ph == kPhSearch && Q.mode <= kTmEnter) Q.node_step(C, st);
+    }
+  }
+}
+
+}  // namespace vt

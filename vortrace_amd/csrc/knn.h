@@ -1,4 +1,4 @@
-// Device k-nearest-neighbour search over the flattened kd-tree.
+// Device k-nearest-neighbour search over the device kd-tree (layout.h).
 //
 // Reproduces nanoflann 1.3.2's KDTreeSingleIndexAdaptor::knnSearch exactly
 // (include/nanoflann.hpp:1221-1258 findNeighbors/knnSearch, 1005-1021
@@ -8,19 +8,24 @@
 // insertion rules, so ids, r^2 bits and the order among equal r^2 match.
 //
 // The recursion becomes a per-lane explicit stack whose first kLdsDepth
-// entries live in LDS (the rest spill to device memory).  Two exact
+// entries live in LDS (the rest spill to device memory).  Three exact
 // transformations keep it short:
 //  * An entry for a far child is pushed only if its box distance is <= the
 //    current worst distance: the worst distance never grows, so any entry
 //    that fails now would fail when nanoflann tests it.
 //  * On the first root-to-leaf descent the worst distance is still infinite.
 //    Its far children go into an LDS ring holding the deepest kLdsDepth
-//    levels; once the first leaf is scanned the ring is compacted to the
-//    entries that can still be visited.  Only if an entry that fell out of
-//    the ring could still be visited is the path walked again.
-// Entries are either PENDING(parent node, box distance of its far child) or
-// UNDO(dim, saved dists[dim]), which restores the per-dimension distance
-// vector exactly where nanoflann's recursion would on return.
+//    levels, which becomes the stack once the first leaf is scanned.  Only if
+//    an entry that fell out of the ring could still be visited is the path
+//    walked again.
+//  * Every entry records a lower bound (a float rounded down) of the box
+//    distances of the entries below it.  Once that bound exceeds the worst
+//    distance, nanoflann would prune everything still pending, so the search
+//    ends there.
+// Entries are PENDING(child reference, box distance of that far child, with
+// the parent's cut dimension) or UNDO(dim, saved dists[dim]), which restores
+// the per-dimension distance vector exactly where nanoflann's recursion would
+// on return.
 #pragma once
 #include "device_common.h"
 
@@ -37,17 +42,20 @@ constexpr int kLdsDepth = 16;
 // so lanes at equal depth touch consecutive banks.
 __shared__ uint32_t lds_stack_tag[kLdsDepth * kBlock];
 __shared__ double lds_stack_val[kLdsDepth * kBlock];
+__shared__ float lds_stack_pb[kLdsDepth * kBlock];
 
 // Per-lane explicit stack.  Logical entries [0, kLdsDepth) live in LDS at slot
-// (base + i) mod kLdsDepth (base is non-zero only after the first descent
-// below, which fills the LDS part as a ring); deeper entries spill to device
-// memory, interleaved by lane so lanes at equal depth coalesce.
+// (base + i) mod kLdsDepth (base is non-zero only after a first descent that
+// filled the LDS part as a ring); deeper entries spill to device memory,
+// interleaved by lane so lanes at equal depth coalesce.  Each entry holds a
+// tag, a value and `pb`, the lower bound of the box distances below it.
 static_assert((kLdsDepth & (kLdsDepth - 1)) == 0, "kLdsDepth must be a power of two");
 
 struct KStack {
   int tid;       // lane within the workgroup
-  uint32_t *gt;  // this lane's spill tags: entry i >= kLdsDepth at gt[(i - kLdsDepth) * lanes]
+  uint32_t *gt;  // this lane's spill: entry i >= kLdsDepth at gt[(i - kLdsDepth) * lanes]
   double *gv;
+  float *gp;
   int lanes, gcap;
   int sp, maxsp;
   bool overflow;
@@ -56,33 +64,47 @@ struct KStack {
   __device__ __forceinline__ int slot(int i) const {
     return ((base + i) & (kLdsDepth - 1)) * kBlock + tid;
   }
-  __device__ __forceinline__ void push(uint32_t tag, double v) {
+  __device__ __forceinline__ void push(uint32_t tag, double v, float pb) {
     if (sp < kLdsDepth) {
-      lds_stack_tag[slot(sp)] = tag;
-      lds_stack_val[slot(sp)] = v;
+      const int sl = slot(sp);
+      lds_stack_tag[sl] = tag;
+      lds_stack_val[sl] = v;
+      lds_stack_pb[sl] = pb;
     } else if (sp - kLdsDepth < gcap) {
-      gst(gt + (size_t)(sp - kLdsDepth) * lanes, tag);
-      gst(gv + (size_t)(sp - kLdsDepth) * lanes, v);
+      const size_t o = (size_t)(sp - kLdsDepth) * lanes;
+      gst(gt + o, tag);
+      gst(gv + o, v);
+      gst(gp + o, pb);
     } else {
       overflow = true;
     }
     ++sp;
     maxsp = sp > maxsp ? sp : maxsp;
   }
-  __device__ __forceinline__ void pop(uint32_t &tag, double &v) {
+  // An entry lost to overflow reads as (kSkip, 0, 0): skipped, and its bound
+  // of 0 stops nothing.
+  __device__ __forceinline__ void pop(uint32_t &tag, double &v, float &pb) {
     --sp;
     if (sp < kLdsDepth) {
-      tag = lds_stack_tag[slot(sp)];
-      v = lds_stack_val[slot(sp)];
+      const int sl = slot(sp);
+      tag = lds_stack_tag[sl];
+      v = lds_stack_val[sl];
+      pb = lds_stack_pb[sl];
     } else if (sp - kLdsDepth < gcap) {
-      tag = gld(gt + (size_t)(sp - kLdsDepth) * lanes);
-      v = gld(gv + (size_t)(sp - kLdsDepth) * lanes);
+      const size_t o = (size_t)(sp - kLdsDepth) * lanes;
+      tag = gld(gt + o);
+      v = gld(gv + o);
+      pb = gld(gp + o);
     } else {
       tag = kSkip;
       v = 0.0;
+      pb = 0.0f;
     }
   }
 };
+
+// Lower bound of a box distance as stored in the stack's `pb` bounds.
+__device__ __forceinline__ float bound_below(double m2) { return __double2float_rd(m2); }
 
 // k <= 3 result set held in registers (KNNResultSet semantics).
 template <int K>
@@ -124,13 +146,16 @@ struct RegRes {
   }
 };
 
-// Result set for a runtime k in {1, 2, 3}, in registers.  Entries at index
-// >= k are never read (worst() is entry k - 1 and add() only compares the
-// first `count` entries), so shifting into them is harmless.
+// Result set for a runtime k in {1, 2, 3}, in registers (the searches of the
+// ray walk).  All three entries are kept sorted: unused entries start at
+// DBL_MAX and entries shifted past position k - 1 are never read, so
+// addPoint's position (after every entry <= dist) is the number of the first
+// two entries <= dist, and `count` is not needed.  w caches worstDist().
 struct RegRes3 {
   double d[3];
   uint32_t id[3];
-  int count, k;
+  int k;
+  double w;
 
   __device__ __forceinline__ void init() {
 #pragma unroll
@@ -138,30 +163,27 @@ struct RegRes3 {
       d[j] = DBL_MAX;
       id[j] = kNone;
     }
-    count = 0;
+    w = DBL_MAX;
   }
-  __device__ __forceinline__ double worst() const {
-    return k == 1 ? d[0] : (k == 2 ? d[1] : d[2]);
-  }
+  __device__ __forceinline__ double worst() const { return w; }
+  // addPoint for dist < worst() (a smaller distance always lands in the first
+  // k entries, at the position after every entry <= dist).
   __device__ __forceinline__ void add(double dist, uint32_t idx) {
-    int pos = 0;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) pos += (j < count && d[j] <= dist) ? 1 : 0;
-    if (pos < k) {
-#pragma unroll
-      for (int j = 2; j > 0; --j)
-        if (j > pos) {
-          d[j] = d[j - 1];
-          id[j] = id[j - 1];
-        }
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        if (j == pos) {
-          d[j] = dist;
-          id[j] = idx;
-        }
-    }
-    if (count < k) ++count;
+    const bool b0 = d[0] <= dist;
+    const bool b1 = d[1] <= dist;  // implies b0
+    const double n2 = b1 ? dist : d[1];
+    const uint32_t j2 = b1 ? idx : id[1];
+    const double n1 = b1 ? d[1] : (b0 ? dist : d[0]);
+    const uint32_t j1 = b1 ? id[1] : (b0 ? idx : id[0]);
+    const double n0 = b0 ? d[0] : dist;
+    const uint32_t j0 = b0 ? id[0] : idx;
+    d[0] = n0;
+    d[1] = n1;
+    d[2] = n2;
+    id[0] = j0;
+    id[1] = j1;
+    id[2] = j2;
+    w = sel3(n0, n1, n2, (uint32_t)(k - 1));
   }
 };
 
@@ -201,14 +223,16 @@ struct MemRes {
   }
 };
 
-// Leaf scan (searchLevel's leaf branch, include/nanoflann.hpp:1349-1366): the
-// worst distance is read once, points are offered in slot order.
+// Leaf scan (searchLevel's leaf branch, include/nanoflann.hpp:1349-1366) of
+// the leaf whose reference is `ref`: the worst distance is read once, points
+// are offered in slot order.
 template <class RS>
 __device__ __forceinline__ void scan_leaf(const DevCloud &C, const double q0, const double q1,
-                                          const double q2, const Node &nd, RS &rs) {
-  const uint32_t end = nd.b & ~kLeafBit;
+                                          const double q2, uint32_t ref, RS &rs) {
+  const uint32_t s0 = ref & kSlotMask;
+  const uint32_t end = (uint32_t)gld(&C.gen[s0].end);
   const double worst = rs.worst();
-  for (uint32_t t = nd.a; t < end; ++t) {
+  for (uint32_t t = s0; t < end; ++t) {
     const double2 *g = reinterpret_cast<const double2 *>(C.gen + t);
     const double2 xy = gld(g);
     const double2 zw = gld(g + 1);
@@ -222,10 +246,9 @@ __device__ __forceinline__ void scan_leaf(const DevCloud &C, const double q0, co
   }
 }
 
-// Child choice at an inner node: returns true when child1 (the lower one,
-// index node + 1) is visited first, and the squared distance to the far
-// child's near bound.
-__device__ __forceinline__ bool choose(const Node &nd, double val, double &cut) {
+// Child choice at an inner node: returns true when child1 is visited first,
+// and the squared distance to the far child's near bound.
+__device__ __forceinline__ bool choose(const DNodeV &nd, double val, double &cut) {
   const double diff1 = val - nd.lo;
   const double diff2 = val - nd.hi;
   const bool low_first = (diff1 + diff2) < 0;
@@ -236,7 +259,7 @@ __device__ __forceinline__ bool choose(const Node &nd, double val, double &cut) 
 
 __device__ __forceinline__ double pick(const double q0, const double q1, const double q2,
                                        uint32_t f) {
-  return f == 0 ? q0 : (f == 1 ? q1 : q2);
+  return sel3(q0, q1, q2, f);
 }
 
 template <class RS, class ST>
@@ -261,68 +284,56 @@ __device__ void knn_core(const DevCloud &C, const double q0, const double q1, co
   // The stack is empty between queries.
   st.sp = 0;
   st.base = 0;
+  float ptop = INFINITY;  // bound of everything on the stack
 
   // Phase 1: best-child descent to the first leaf.  The far child of every
-  // level (and its box distance) goes into the LDS ring at slot depth mod
-  // kLdsDepth; the smallest box distance that falls out of the ring is kept.
-  uint32_t node = 0;
-  Node nd = load_node(C.nodes, 0);
+  // level goes into the LDS ring at slot depth mod kLdsDepth; the smallest
+  // box distance that falls out of the ring is kept.
+  uint32_t ref = C.root;
   int depth = 0;
   double lost_min = DBL_MAX;
-  while (!(nd.b & kLeafBit)) {
+  while (!(ref & kRefLeaf)) {
+    const DNodeV nd = load_node(C.nodes, ref);
     double cut;
-    const uint32_t f = nd.b;
-    const bool low_first = choose(nd, pick(q0, q1, q2, f), cut);
-    const double m2 = mind + cut - dd[f];
+    const bool low_first = choose(nd, pick(q0, q1, q2, nd.dim), cut);
+    const double m2 = mind + cut - sel3(dd[0], dd[1], dd[2], nd.dim);
     const int sl = (depth & (kLdsDepth - 1)) * kBlock + st.tid;
     if (depth >= kLdsDepth) {
       const double old = lds_stack_val[sl];
       lost_min = old < lost_min ? old : lost_min;
     }
-    lds_stack_tag[sl] = node;
+    lds_stack_tag[sl] = far_tag(low_first ? nd.c2 : nd.c1, nd.dim);
     lds_stack_val[sl] = m2;
+    lds_stack_pb[sl] = ptop;
+    ptop = fminf(ptop, bound_below(m2));
     ++depth;
-    node = low_first ? node + 1 : nd.a;
-    nd = load_node(C.nodes, node);
+    ref = low_first ? nd.c1 : nd.c2;
   }
-  const uint32_t first_leaf = node;
-  scan_leaf(C, q0, q1, q2, nd, rs);
+  const uint32_t first_leaf = ref;
+  scan_leaf(C, q0, q1, q2, ref, rs);
 
   const double worst0 = rs.worst();
   if (depth <= kLdsDepth || !(lost_min <= worst0)) {
     // Every entry that left the ring would be pruned (the worst distance
-    // never grows): keep the ring's entries that can still be visited, in
-    // order, as the stack.  Compaction writes only to slots already read.
-    const int n = depth < kLdsDepth ? depth : kLdsDepth;
-    const int base = depth > kLdsDepth ? (depth & (kLdsDepth - 1)) : 0;
-    st.base = base;
-    int w = 0;
-    for (int m = 0; m < n; ++m) {
-      const int from = ((base + m) & (kLdsDepth - 1)) * kBlock + st.tid;
-      const double v = lds_stack_val[from];
-      if (v <= worst0) {
-        const uint32_t t = lds_stack_tag[from];
-        const int to = ((base + w) & (kLdsDepth - 1)) * kBlock + st.tid;
-        lds_stack_tag[to] = t;
-        lds_stack_val[to] = v;
-        ++w;
-      }
-    }
-    st.sp = w;
-    st.maxsp = w > st.maxsp ? w : st.maxsp;
+    // never grows): the ring is the stack.
+    st.sp = depth < kLdsDepth ? depth : kLdsDepth;
+    st.base = depth > kLdsDepth ? (depth & (kLdsDepth - 1)) : 0;
+    st.maxsp = st.sp > st.maxsp ? st.sp : st.maxsp;
   } else {
     // An entry that fell out of the ring can still be visited: walk the
     // same path again and push every far child that can still be visited.
-    node = 0;
-    nd = load_node(C.nodes, 0);
-    while (node != first_leaf) {
+    ptop = INFINITY;
+    ref = C.root;
+    while (ref != first_leaf) {
+      const DNodeV nd = load_node(C.nodes, ref);
       double cut;
-      const uint32_t f = nd.b;
-      const bool low_first = choose(nd, pick(q0, q1, q2, f), cut);
-      const double m2 = mind + cut - dd[f];
-      if (m2 <= worst0) st.push(node, m2);
-      node = low_first ? node + 1 : nd.a;
-      nd = load_node(C.nodes, node);
+      const bool low_first = choose(nd, pick(q0, q1, q2, nd.dim), cut);
+      const double m2 = mind + cut - sel3(dd[0], dd[1], dd[2], nd.dim);
+      if (m2 <= worst0) {
+        st.push(far_tag(low_first ? nd.c2 : nd.c1, nd.dim), m2, ptop);
+        ptop = fminf(ptop, bound_below(m2));
+      }
+      ref = low_first ? nd.c1 : nd.c2;
     }
   }
 
@@ -330,38 +341,52 @@ __device__ void knn_core(const DevCloud &C, const double q0, const double q1, co
   while (st.sp > 0) {
     uint32_t tag;
     double v;
-    st.pop(tag, v);
+    float pb;
+    st.pop(tag, v, pb);
+    ptop = pb;
     if (tag == kSkip) continue;
-    if (tag & kUndo) {
-      dd[tag & 3u] = v;
-      continue;
-    }
-    if (!(v <= rs.worst())) continue;  // mindistsq * epsError <= worstDist()
-    const Node pn = load_node(C.nodes, tag);
-    const uint32_t f = pn.b;
-    double cut;
-    const bool low_first = choose(pn, pick(q0, q1, q2, f), cut);
-    node = low_first ? pn.a : tag + 1;  // the far child
-    nd = load_node(C.nodes, node);
-    if (nd.b & kLeafBit) {
-      scan_leaf(C, q0, q1, q2, nd, rs);
-      continue;
-    }
-    st.push(kUndo | f, dd[f]);
-    dd[f] = cut;
-    mind = v;
     const double worst = rs.worst();
-    while (!(nd.b & kLeafBit)) {
-      double c2;
-      const uint32_t f2 = nd.b;
-      const bool lf = choose(nd, pick(q0, q1, q2, f2), c2);
-      const double m2 = mind + c2 - dd[f2];
-      if (m2 <= worst) st.push(node, m2);
-      node = lf ? node + 1 : nd.a;
-      nd = load_node(C.nodes, node);
+    if ((tag & kUndo) == kUndo) {
+      const uint32_t f = tag & 3u;
+      dd[0] = f == 0 ? v : dd[0];
+      dd[1] = f == 1 ? v : dd[1];
+      dd[2] = f == 2 ? v : dd[2];
+    } else if (v <= worst) {  // mindistsq * epsError <= worstDist()
+      if (tag & kRefLeaf) {
+        scan_leaf(C, q0, q1, q2, tag, rs);
+        continue;
+      }
+      // Entering an inner far child: dists[idx] = cut_dist, mindistsq from
+      // the entry; the old dists[idx] is restored by the undo entry.
+      ref = tag & kIdxMask;
+      const uint32_t pdim = (tag >> 29) & 3u;
+      DNodeV nd = load_node(C.nodes, ref);
+      const double c = pick(q0, q1, q2, pdim) - nd.pb;
+      const double dold = sel3(dd[0], dd[1], dd[2], pdim);
+      st.push(kUndo | pdim, dold, ptop);
+      dd[0] = pdim == 0 ? c * c : dd[0];
+      dd[1] = pdim == 1 ? c * c : dd[1];
+      dd[2] = pdim == 2 ? c * c : dd[2];
+      mind = v;
+      for (;;) {
+        double cut;
+        const bool low_first = choose(nd, pick(q0, q1, q2, nd.dim), cut);
+        const double m2 = mind + cut - sel3(dd[0], dd[1], dd[2], nd.dim);
+        if (m2 <= worst) {
+          st.push(far_tag(low_first ? nd.c2 : nd.c1, nd.dim), m2, ptop);
+          ptop = fminf(ptop, bound_below(m2));
+        }
+        ref = low_first ? nd.c1 : nd.c2;
+        if (ref & kRefLeaf) break;
+        nd = load_node(C.nodes, ref);
+      }
+      scan_leaf(C, q0, q1, q2, ref, rs);
+      continue;
     }
-    scan_leaf(C, q0, q1, q2, nd, rs);
+    // Nothing below can be visited any more: nanoflann prunes all of it.
+    if ((double)pb > worst) break;
   }
+  st.sp = 0;
 }
 
 // PointCloud::minDistSqToBox, src/pointcloud.cpp:283-298.
@@ -521,96 +546,6 @@ __device__ void query_reg(const DevCloud &C, const double q0, const double q1, c
   for (int j = 0; j < K; ++j) {
     ids[j] = mi[j];
     r2[j] = md[j];
-  }
-}
-
-// queryTreeK for a runtime k in {1, 2, 3} (src/pointcloud.cpp:308-368): one
-// code path for every query of the walk.  Same merge as query_reg.
-template <bool kPeriodic, class ST>
-__device__ __forceinline__ void query_rt(const DevCloud &C, const double q0, const double q1,
-                                         const double q2, int k, uint32_t (&ids)[3],
-                                         double (&r2)[3], ST &st) {
-  RegRes3 rs;
-  rs.k = k;
-  knn_core(C, q0, q1, q2, rs, st);
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    ids[j] = j < k ? rs.id[j] : kNone;
-    r2[j] = j < k ? rs.d[j] : DBL_MAX;
-  }
-  if constexpr (!kPeriodic) return;
-  const double qv[3] = {q0, q1, q2};
-  {
-    ImageBox ib;
-    ib.init(C, qv);
-    if (!ib.any_within(rs.worst() + kTol)) return;
-  }
-  uint32_t mi[6];
-  double md[6];
-  int m = k;
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    mi[j] = rs.id[j];
-    md[j] = rs.d[j];
-  }
-#pragma unroll
-  for (int j = 3; j < 6; ++j) {
-    mi[j] = kNone;
-    md[j] = DBL_MAX;
-  }
-  for (int ix = -1; ix <= 1; ix++)
-    for (int iy = -1; iy <= 1; iy++)
-      for (int iz = -1; iz <= 1; iz++) {
-        if (ix == 0 && iy == 0 && iz == 0) continue;
-        const double sh[3] = {qv[0] + ix * C.box[0], qv[1] + iy * C.box[1],
-                              qv[2] + iz * C.box[2]};
-        const double wk = k == 1 ? md[0] : (k == 2 ? md[1] : md[2]);
-        if (min_dist_sq_to_box(C, sh) > wk + kTol) continue;
-        RegRes3 ri;
-        ri.k = k;
-        knn_core(C, sh[0], sh[1], sh[2], ri, st);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          if (i < k) {
-            bool found = false;
-#pragma unroll
-            for (int j = 0; j < 6; ++j) {
-              if (!found && j < m && mi[j] == ri.id[i]) {
-                md[j] = (ri.d[i] < md[j]) ? ri.d[i] : md[j];
-                found = true;
-              }
-            }
-            if (!found) {
-#pragma unroll
-              for (int j = 0; j < 6; ++j)
-                if (j == m) {
-                  mi[j] = ri.id[i];
-                  md[j] = ri.d[i];
-                }
-              ++m;
-            }
-          }
-        }
-#pragma unroll
-        for (int i = 1; i < 6; ++i) {
-#pragma unroll
-          for (int j = i; j > 0; --j) {
-            if (i < m && md[j] < md[j - 1]) {
-              const double td = md[j];
-              md[j] = md[j - 1];
-              md[j - 1] = td;
-              const uint32_t ti = mi[j];
-              mi[j] = mi[j - 1];
-              mi[j - 1] = ti;
-            }
-          }
-        }
-        if (m > k) m = k;
-      }
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    ids[j] = j < k ? mi[j] : kNone;
-    r2[j] = j < k ? md[j] : DBL_MAX;
   }
 }
 

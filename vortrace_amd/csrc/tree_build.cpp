@@ -130,8 +130,7 @@ void build_rec(const Ctx &x, size_t l, size_t r, Box &box, int64_t depth, SubTre
     n.lo = n.hi = 0.0;
     n.a = static_cast<uint32_t>(l);
     n.b = static_cast<uint32_t>(r) | kLeafBit;
-    n.depth = static_cast<uint32_t>(depth);
-    n.pad = 0;
+    n.pb = 0.0;
     leaf_box(x, l, r, box);
     out.leaves++;
     return;
@@ -166,8 +165,9 @@ void build_rec(const Ctx &x, size_t l, size_t r, Box &box, int64_t depth, SubTre
   n.hi = rb.lo[f];
   n.a = child2;
   n.b = static_cast<uint32_t>(f);
-  n.depth = static_cast<uint32_t>(depth);
-  n.pad = 0;
+  n.pb = 0.0;
+  out.nodes[me + 1].pb = n.lo;
+  out.nodes[child2].pb = n.hi;
   for (int d = 0; d < 3; ++d) {
     box.lo[d] = std::min(lb.lo[d], rb.lo[d]);
     box.hi[d] = std::max(lb.hi[d], rb.hi[d]);
@@ -204,6 +204,39 @@ void build_tree(const double *pts, size_t n, int threads, HostTree &out) {
   out.nodes = std::move(st.nodes);
   out.leaves = st.leaves;
   out.max_depth = st.max_depth;
+}
+
+void make_device_tree(const HostTree &t, DeviceTree &out) {
+  const size_t nh = t.nodes.size();
+  const size_t n = t.vind.size();
+  if (n >= kMaxSlots) throw std::invalid_argument("too many points for one cloud (2^30 slots)");
+  std::vector<uint32_t> dev(nh, 0);  // host node -> device index (inner nodes)
+  uint32_t ninner = 0;
+  for (size_t h = 0; h < nh; ++h)
+    if (!(t.nodes[h].b & kLeafBit)) dev[h] = ninner++;
+  if (ninner >= kMaxNodes)
+    throw std::invalid_argument("too many points for one cloud (2^29 inner nodes)");
+  auto ref = [&](size_t h) -> uint32_t {
+    const Node &c = t.nodes[h];
+    return (c.b & kLeafBit) ? (kRefLeaf | c.a) : dev[h];
+  };
+  out.nodes.resize(ninner);
+  out.leaf_end.assign(n, 0);
+  for (size_t h = 0; h < nh; ++h) {
+    const Node &nd = t.nodes[h];
+    if (nd.b & kLeafBit) {
+      const uint32_t end = nd.b & ~kLeafBit;
+      for (uint32_t s = nd.a; s < end; ++s) out.leaf_end[s] = end;
+      continue;
+    }
+    DNode &d = out.nodes[dev[h]];
+    d.lo = nd.lo;
+    d.hi = nd.hi;
+    d.pb = nd.pb;
+    d.c1 = ref(h + 1) | ((nd.b & 1u) ? kDimBit : 0u);
+    d.c2 = ref(nd.a) | ((nd.b & 2u) ? kDimBit : 0u);
+  }
+  out.root = ref(0);
 }
 
 }  // namespace vt

@@ -22,4 +22,14 @@ struct HostTree {
 // reference's.  Uses up to `threads` host threads for independent subtrees.
 void build_tree(const double *pts, size_t n, int threads, HostTree &out);
 
+// The device tree: the inner nodes of t in pre-order as DNodes, the root's
+// reference, and for every slot the end slot of its leaf.  Throws
+// std::invalid_argument when the tree exceeds the reference encoding.
+struct DeviceTree {
+  std::vector<DNode> nodes;
+  std::vector<uint64_t> leaf_end;
+  uint32_t root = 0;
+};
+void make_device_tree(const HostTree &t, DeviceTree &out);
+
 }  // namespace vt

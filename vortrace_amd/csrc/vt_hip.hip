@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -27,7 +28,7 @@
 
 #include "../../include/vortrace_hip.h"
 #include "tree_build.h"
-#include "walk.h"
+#include "engine.h"
 
 #pragma clang fp contract(off)
 
@@ -157,11 +158,107 @@ struct FillSink {  // segments at the ray's offset, plus the reduced values
   __device__ void store(double *out) const { in.store(out); }
 };
 
-// -------------------------------------------------------------- kernels ----
+// ------------------------------------------------------------ ray I/O ----
+// Per-ray sink set-up and output for run_engine (engine.h).
 
-struct LaneState {
-  __device__ static int gtid() { return blockIdx.x * kBlock + threadIdx.x; }
+template <int NF>
+struct SumIO {
+  using Sink = SumSink<NF>;
+  const double *fields;
+  double *out;
+  int nout;
+  __device__ void begin(int64_t, Sink &sk) const {
+    sk.fields = fields;
+    sk.init();
+  }
+  __device__ int end(int64_t ray, Sink &sk) const {
+    sk.store(out + ray * nout);
+    return kErrNone;
+  }
 };
+
+struct SumDynIO {
+  using Sink = SumSinkDyn;
+  const double *fields;
+  int nf;
+  double *out;
+  int nout;
+  __device__ void begin(int64_t ray, Sink &sk) const {
+    sk.fields = fields;
+    sk.nf = nf;
+    sk.out = out + ray * nout;
+    sk.init();
+  }
+  __device__ int end(int64_t, Sink &) const { return kErrNone; }
+};
+
+template <bool kMax>
+struct ExtIO {
+  using Sink = ExtSink<kMax>;
+  const double *fields;
+  int nf;
+  double *out;
+  int nout;
+  __device__ void begin(int64_t ray, Sink &sk) const {
+    sk.fields = fields;
+    sk.nf = nf;
+    sk.out = out + ray * nout;
+    sk.init();
+  }
+  __device__ int end(int64_t, Sink &) const { return kErrNone; }
+};
+
+struct VolIO {
+  using Sink = VolSink;
+  const double *fields;
+  double *out;
+  int nout;
+  __device__ void begin(int64_t, Sink &sk) const {
+    sk.fields = fields;
+    sk.init();
+  }
+  __device__ int end(int64_t ray, Sink &sk) const {
+    sk.store(out + ray * nout);
+    return kErrNone;
+  }
+};
+
+struct CountIO {
+  using Sink = CountSink;
+  int64_t *counts;
+  __device__ void begin(int64_t, Sink &sk) const { sk.init(); }
+  __device__ int end(int64_t ray, Sink &sk) const {
+    gst(counts + ray, (int64_t)sk.n);
+    return kErrNone;
+  }
+};
+
+// Segments at the ray's offset (from the count pass) plus the reduced values.
+template <class InnerIO>
+struct FillIO {
+  using Sink = FillSink<typename InnerIO::Sink>;
+  InnerIO inner;
+  const int64_t *offsets;
+  const uint32_t *slot_id;
+  int64_t *ids;
+  double *se, *sx;
+  __device__ void begin(int64_t ray, Sink &sk) const {
+    const int64_t off = gld(offsets + ray);
+    sk.slot_id = slot_id;
+    sk.ids = ids + off;
+    sk.se = se + off;
+    sk.sx = sx + off;
+    sk.cap = (uint64_t)(gld(offsets + ray + 1) - off);
+    inner.begin(ray, sk.in);
+    sk.n = 0;
+  }
+  __device__ int end(int64_t ray, Sink &sk) const {
+    inner.end(ray, sk.in);
+    return sk.n == sk.cap ? kErrNone : kErrFillOverflow;
+  }
+};
+
+// -------------------------------------------------------------- kernels ----
 
 __device__ __forceinline__ void flush_stats(Stats *stats, uint64_t nseg, uint64_t nq,
                                             uint64_t nslow, int maxw, int maxk) {
@@ -176,132 +273,92 @@ __device__ __forceinline__ void flush_stats(Stats *stats, uint64_t nseg, uint64_
   const int tid = threadIdx.x;                                                  \
   const int gtid = blockIdx.x * kBlock + tid;                                   \
   const LaneInfo lane_info{tid, gtid, gtid >> 6, (int)(tid & 63)};              \
-  KStack st = make_stack(S, lane_info);                                         \
-  WalkCounters cnt;
+  KStack st = make_stack(S, lane_info);
+
+template <bool PER, bool EXP, class IO>
+__device__ __forceinline__ void run_rays(const DevCloud &C, const Scratch &S, const double *start,
+                                         const double *end, int64_t nrays,
+                                         const uint32_t *ray_list, uint32_t *defer, ErrRec *err,
+                                         Stats *stats, IO &io) {
+  VT_LANE_SETUP(PER)
+  (void)lane_info;
+  EngineCounters cnt;
+  run_engine<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io, st, cnt);
+  flush_stats(stats, cnt.nseg, cnt.nq + cnt.nslow, cnt.nslow, cnt.maxw, st.maxsp);
+}
+
+// Every kernel below runs the rays [0, nrays) (or ray_list[0, nrays)).  The
+// fast variants (EXP = false) append rays that need the K_EXPAND
+// classification to `defer`; the EXP = true variants handle every ray.
 
 template <int MODE, int NF, bool PER>
 __global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const double *start,
                                                     const double *end, int64_t nrays,
-                                                    double *out, int nout, ErrRec *err,
-                                                    Stats *stats) {
-  VT_LANE_SETUP(PER)
-  uint64_t nseg = 0;
-  int maxw = 0;
-  for (;;) {
-    const int64_t ray = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
-    if (ray >= nrays) break;
-    if (__hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
-    RayGeom r;
-    make_ray(start + 3 * ray, end + 3 * ray, r);
-    double ev[3] = {0, 0, 0};
-    uint32_t ei[2] = {0, 0};
-    int rc;
-    double *o = out + ray * nout;
-    if constexpr (MODE == VT_SUM && NF > 0) {
-      SumSink<NF> sk{C.fields};
-      sk.init();
-      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
-      sk.store(o);
-      nseg += sk.n;
-    } else if constexpr (MODE == VT_SUM) {
-      SumSinkDyn sk{C.fields, (int)C.nf, o};
-      sk.init();
-      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
-      nseg += sk.n;
-    } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
-      ExtSink<MODE == VT_MAX> sk{C.fields, (int)C.nf, o};
-      sk.init();
-      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
-      nseg += sk.n;
-    } else {
-      VolSink sk{C.fields};
-      sk.init();
-      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
-      sk.store(o);
-      nseg += sk.n;
-    }
-    if (!rc && st.overflow) rc = kErrKnnStack;
-    if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
+                                                    uint32_t *defer, double *out, int nout,
+                                                    ErrRec *err, Stats *stats) {
+  if constexpr (MODE == VT_SUM && NF > 0) {
+    SumIO<NF> io{C.fields, out, nout};
+    run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
+  } else if constexpr (MODE == VT_SUM) {
+    SumDynIO io{C.fields, (int)C.nf, out, nout};
+    run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
+  } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
+    ExtIO<MODE == VT_MAX> io{C.fields, (int)C.nf, out, nout};
+    run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
+  } else {
+    VolIO io{C.fields, out, nout};
+    run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
   }
-  flush_stats(stats, nseg, cnt.nq, cnt.nslow, maxw, st.maxsp);
 }
 
-template <bool PER>
+// The deferred rays of k_project.  Sums accumulate in place, in the same order
+// as SumSink's registers, so the values are the same bits.
+template <int MODE, bool PER>
+__global__ __launch_bounds__(kBlock) void k_project_x(DevCloud C, Scratch S,
+                                                      const double *start, const double *end,
+                                                      int64_t nrays, const uint32_t *ray_list,
+                                                      double *out, int nout, ErrRec *err,
+                                                      Stats *stats) {
+  if constexpr (MODE == VT_SUM) {
+    SumDynIO io{C.fields, (int)C.nf, out, nout};
+    run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
+  } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
+    ExtIO<MODE == VT_MAX> io{C.fields, (int)C.nf, out, nout};
+    run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
+  } else {
+    VolIO io{C.fields, out, nout};
+    run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
+  }
+}
+
+template <bool PER, bool EXP>
 __global__ __launch_bounds__(kBlock) void k_count(DevCloud C, Scratch S, const double *start,
                                                   const double *end, int64_t nrays,
+                                                  const uint32_t *ray_list, uint32_t *defer,
                                                   int64_t *counts, ErrRec *err, Stats *stats) {
-  VT_LANE_SETUP(PER)
-  uint64_t nseg = 0;
-  int maxw = 0;
-  for (;;) {
-    const int64_t ray = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
-    if (ray >= nrays) break;
-    if (__hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
-    RayGeom r;
-    make_ray(start + 3 * ray, end + 3 * ray, r);
-    double ev[3] = {0, 0, 0};
-    uint32_t ei[2] = {0, 0};
-    CountSink sk;
-    sk.init();
-    int rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
-    if (!rc && st.overflow) rc = kErrKnnStack;
-    gst(counts + ray, (int64_t)sk.n);
-    nseg += sk.n;
-    if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
-  }
-  flush_stats(stats, nseg, cnt.nq, cnt.nslow, maxw, st.maxsp);
+  CountIO io{counts};
+  run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
 }
 
-template <int MODE, bool PER>
+template <int MODE, bool PER, bool EXP>
 __global__ __launch_bounds__(kBlock) void k_fill(DevCloud C, Scratch S, const double *start,
                                                  const double *end, int64_t nrays,
+                                                 const uint32_t *ray_list, uint32_t *defer,
                                                  const int64_t *offsets, double *out, int nout,
                                                  const uint32_t *slot_id, int64_t *ids,
                                                  double *se, double *sx, ErrRec *err,
                                                  Stats *stats) {
-  VT_LANE_SETUP(PER)
-  uint64_t nseg = 0;
-  int maxw = 0;
-  for (;;) {
-    const int64_t ray = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
-    if (ray >= nrays) break;
-    if (__hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) continue;
-    RayGeom r;
-    make_ray(start + 3 * ray, end + 3 * ray, r);
-    double ev[3] = {0, 0, 0};
-    uint32_t ei[2] = {0, 0};
-    const int64_t off = gld(offsets + ray);
-    const int64_t cap = gld(offsets + ray + 1) - off;
-    double *o = out + ray * nout;
-    int rc;
-    uint64_t n;
-    if constexpr (MODE == VT_SUM) {
-      FillSink<SumSinkDyn> sk{SumSinkDyn{C.fields, (int)C.nf, o}, slot_id, ids + off, se + off,
-                              sx + off, (uint64_t)cap};
-      sk.init();
-      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
-      n = sk.n;
-    } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
-      FillSink<ExtSink<MODE == VT_MAX>> sk{ExtSink<MODE == VT_MAX>{C.fields, (int)C.nf, o},
-                                           slot_id, ids + off, se + off, sx + off,
-                                           (uint64_t)cap};
-      sk.init();
-      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
-      n = sk.n;
-    } else {
-      FillSink<VolSink> sk{VolSink{C.fields}, slot_id, ids + off, se + off, sx + off,
-                           (uint64_t)cap};
-      sk.init();
-      rc = walk_ray<PER>(C, S, lane_info, st, r, sk, maxw, cnt, ev, ei);
-      sk.store(o);
-      n = sk.n;
-    }
-    nseg += n;
-    if (!rc && st.overflow) rc = kErrKnnStack;
-    if (!rc && (int64_t)n != cap) rc = kErrFillOverflow;
-    if (rc) report_error(err, rc, ray, ev[0], ev[1], ev[2], ei[0], ei[1]);
+  if constexpr (MODE == VT_SUM) {
+    FillIO<SumDynIO> io{SumDynIO{C.fields, (int)C.nf, out, nout}, offsets, slot_id, ids, se, sx};
+    run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
+  } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
+    FillIO<ExtIO<MODE == VT_MAX>> io{ExtIO<MODE == VT_MAX>{C.fields, (int)C.nf, out, nout},
+                                     offsets, slot_id, ids, se, sx};
+    run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
+  } else {
+    FillIO<VolIO> io{VolIO{C.fields, out, nout}, offsets, slot_id, ids, se, sx};
+    run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
   }
-  flush_stats(stats, nseg, cnt.nq, cnt.nslow, maxw, st.maxsp);
 }
 
 template <int K, bool PER>
@@ -425,7 +482,7 @@ struct vt_cloud {
 
   // device copy
   int device = -1;
-  Node *d_nodes = nullptr;
+  DNode *d_nodes = nullptr;
   Gen *d_gen = nullptr;
   double *d_fields = nullptr;
   uint32_t *d_slot_id = nullptr;
@@ -433,15 +490,20 @@ struct vt_cloud {
   Stats *d_stats = nullptr;
   Scratch scratch{};
   size_t scratch_lanes = 0;
+  uint32_t *d_defer = nullptr;  // rays left to the K_EXPAND-capable kernels
+  size_t defer_cap = 0;
   int kspill = 0;
   DevCloud dc{};
 
   void free_device() {
     for (void *p : {(void *)d_nodes, (void *)d_gen, (void *)d_fields, (void *)d_slot_id,
                     (void *)d_err, (void *)d_stats, (void *)scratch.knn_tag,
-                    (void *)scratch.knn_val, (void *)scratch.walk_tag, (void *)scratch.walk_s,
-                    (void *)scratch.big_id, (void *)scratch.big_d})
+                    (void *)scratch.knn_val, (void *)scratch.knn_pb, (void *)scratch.walk_tag,
+                    (void *)scratch.walk_s,
+                    (void *)scratch.big_id, (void *)scratch.big_d, (void *)d_defer})
       if (p) (void)hipFree(p);
+    d_defer = nullptr;
+    defer_cap = 0;
     d_nodes = nullptr;
     d_gen = nullptr;
     d_fields = nullptr;
@@ -494,9 +556,28 @@ int grid_for(K kernel, int64_t items, int dev) {
 }
 
 // Per-lane device scratch for `lanes` persistent lanes.
+// Lanes of a wave that run their walk steps together (engine.h); VT_WALK_BATCH
+// overrides the default.
+int env_batch(const char *name, int dflt) {
+  const char *e = std::getenv(name);
+  const int b = e ? std::atoi(e) : 0;
+  return (b >= 1 && b <= 64) ? b : dflt;
+}
+int walk_batch() {
+  static const int v = env_batch("VT_WALK_BATCH", 32);
+  return v;
+}
+// Lanes of a wave that run their leaf steps together; VT_LEAF_BATCH overrides.
+int leaf_batch() {
+  static const int v = env_batch("VT_LEAF_BATCH", 24);
+  return v;
+}
+
 void ensure_scratch(vt_cloud *c, size_t lanes) {
+  c->scratch.walk_batch = walk_batch();
+  c->scratch.leaf_batch = leaf_batch();
   if (lanes <= c->scratch_lanes) return;
-  for (void *p : {(void *)c->scratch.knn_tag, (void *)c->scratch.knn_val,
+  for (void *p : {(void *)c->scratch.knn_tag, (void *)c->scratch.knn_val, (void *)c->scratch.knn_pb,
                   (void *)c->scratch.walk_tag, (void *)c->scratch.walk_s,
                   (void *)c->scratch.big_id, (void *)c->scratch.big_d})
     if (p) (void)hipFree(p);
@@ -506,13 +587,25 @@ void ensure_scratch(vt_cloud *c, size_t lanes) {
   s.lanes = (int)lanes;
   s.knn_tag = dev_alloc<uint32_t>((size_t)s.kspill * lanes);
   s.knn_val = dev_alloc<double>((size_t)s.kspill * lanes);
+  s.knn_pb = dev_alloc<float>((size_t)s.kspill * lanes);
   s.walk_tag = dev_alloc<uint32_t>((size_t)s.wcap * lanes);
   s.walk_s = dev_alloc<double>((size_t)s.wcap * lanes);
   const size_t waves = (lanes + 63) / 64;
   s.big_id = dev_alloc<uint32_t>(waves * 4 * kMaxQueryK);
   s.big_d = dev_alloc<double>(waves * 4 * kMaxQueryK);
+  s.walk_batch = walk_batch();
+  s.leaf_batch = leaf_batch();
   c->scratch = s;
   c->scratch_lanes = lanes;
+}
+
+void ensure_defer(vt_cloud *c, size_t nrays) {
+  if (nrays <= c->defer_cap) return;
+  if (c->d_defer) (void)hipFree(c->d_defer);
+  c->d_defer = nullptr;
+  c->defer_cap = 0;
+  c->d_defer = dev_alloc<uint32_t>(nrays);
+  c->defer_cap = nrays;
 }
 
 void reset_call(vt_cloud *c, hipStream_t st) {
@@ -522,7 +615,8 @@ void reset_call(vt_cloud *c, hipStream_t st) {
 
 // Waits for the call, copies the error record and statistics back and turns
 // a kernel error into the reference's message.
-int finish_call(vt_cloud *c, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+// With `more`, the statistics are added to those of the call's first launch.
+int finish_call(vt_cloud *c, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool more = false) {
   ErrRec er;
   Stats ss;
   HIP_OK(hipMemcpyAsync(&er, c->d_err, sizeof er, hipMemcpyDeviceToHost, st));
@@ -530,12 +624,14 @@ int finish_call(vt_cloud *c, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
   HIP_OK(hipStreamSynchronize(st));
   float ms = 0;
   if (e0 && e1) HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-  g_stats.segments = (int64_t)ss.segments;
-  g_stats.queries = (int64_t)ss.queries;
-  g_stats.max_walk_depth = ss.max_walk_depth;
-  g_stats.max_knn_depth = ss.max_knn_depth;
-  g_stats.slow_queries = (int64_t)ss.slow_queries;
-  g_stats.kernel_ms = ms;
+  if (!more) g_stats = vt_call_stats{};
+  g_stats.segments += (int64_t)ss.segments;
+  g_stats.queries += (int64_t)ss.queries;
+  g_stats.max_walk_depth = std::max<int64_t>(g_stats.max_walk_depth, ss.max_walk_depth);
+  g_stats.max_knn_depth = std::max<int64_t>(g_stats.max_knn_depth, ss.max_knn_depth);
+  g_stats.slow_queries += (int64_t)ss.slow_queries;
+  g_stats.deferred_rays += (int64_t)ss.deferred;
+  g_stats.kernel_ms += ms;
   if (!er.flag) return VT_OK;
   auto cell = [&](uint32_t slot) -> size_t {
     return slot < c->tree.vind.size() ? (size_t)c->tree.vind[slot] : (size_t)slot;
@@ -616,25 +712,58 @@ int guarded(F &&f) {
 
 namespace {
 
-template <int MODE, int NF, bool PER>
-void launch_project_p(vt_cloud *c, const double *s, const double *e, int64_t n, double *out,
-                      int nout, hipStream_t st, Timer &tm) {
-  const int grid = grid_for(k_project<MODE, NF, PER>, n, c->device);
+// One call = the fast kernel over every ray, then, if it deferred any ray
+// (K_EXPAND), the K_EXPAND-capable kernel over the deferred rays.
+// fast(timer) and slow(timer, n_deferred) launch and record the timer's events.
+template <class Fast, class Slow>
+int two_pass(vt_cloud *c, hipStream_t st, size_t nrays, Fast &&fast, Slow &&slow) {
+  if (nrays > 0xFFFFFFFFull) return fail(VT_EINVAL, "at most 2^32 - 1 rays per call");
+  ensure_defer(c, nrays);
+  reset_call(c, st);
+  {
+    Timer tm;
+    fast(tm);
+    const int rc = finish_call(c, st, tm.e0, tm.e1);
+    if (rc) return rc;
+  }
+  const int64_t nd = g_stats.deferred_rays;
+  if (nd == 0) return VT_OK;
+  reset_call(c, st);
+  Timer tm;
+  slow(tm, nd);
+  return finish_call(c, st, tm.e0, tm.e1, true);
+}
+
+template <class K, class... A>
+void launch(vt_cloud *c, K kernel, int64_t n, hipStream_t st, Timer &tm, A... args) {
+  const int grid = grid_for(kernel, n, c->device);
   ensure_scratch(c, (size_t)grid * kBlock);
   HIP_OK(hipEventRecord(tm.e0, st));
-  k_project<MODE, NF, PER><<<grid, kBlock, 0, st>>>(c->dc, c->scratch, s, e, n, out, nout,
-                                                    c->d_err, c->d_stats);
+  kernel<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, args...);
   HIP_OK(hipGetLastError());
   HIP_OK(hipEventRecord(tm.e1, st));
 }
 
+template <int MODE, int NF, bool PER>
+int project_p(vt_cloud *c, const double *s, const double *e, int64_t n, double *out, int nout,
+              hipStream_t st) {
+  return two_pass(
+      c, st, (size_t)n,
+      [&](Timer &tm) {
+        launch(c, k_project<MODE, NF, PER>, n, st, tm, s, e, n, c->d_defer, out, nout, c->d_err,
+               c->d_stats);
+      },
+      [&](Timer &tm, int64_t nd) {
+        launch(c, k_project_x<MODE, PER>, nd, st, tm, s, e, nd, (const uint32_t *)c->d_defer,
+               out, nout, c->d_err, c->d_stats);
+      });
+}
+
 template <int MODE, int NF>
-void launch_project(vt_cloud *c, const double *s, const double *e, int64_t n, double *out,
-                    int nout, hipStream_t st, Timer &tm) {
-  if (c->periodic)
-    launch_project_p<MODE, NF, true>(c, s, e, n, out, nout, st, tm);
-  else
-    launch_project_p<MODE, NF, false>(c, s, e, n, out, nout, st, tm);
+int project(vt_cloud *c, const double *s, const double *e, int64_t n, double *out, int nout,
+            hipStream_t st) {
+  return c->periodic ? project_p<MODE, NF, true>(c, s, e, n, out, nout, st)
+                     : project_p<MODE, NF, false>(c, s, e, n, out, nout, st);
 }
 
 int check_mode(const vt_cloud *c, int mode) {
@@ -795,27 +924,30 @@ int vt_cloud_upload(vt_cloud *c) {
     if (!c->host_tree)
       return fail(VT_ERUNTIME, "There is currently no valid tree for this point cloud");
     const HostTree &t = c->tree;
-    // Slot-ordered generators and fields.
+    DeviceTree dt;
+    make_device_tree(t, dt);
+    // Slot-ordered generators (with their leaf's end slot) and fields.
     std::vector<Gen> gen(c->n);
     std::vector<double> fslot(c->n * c->nf);
     for (size_t s = 0; s < c->n; ++s) {
       const size_t i = t.vind[s];
-      gen[s] = Gen{c->pts[3 * i], c->pts[3 * i + 1], c->pts[3 * i + 2], 0.0};
+      gen[s] = Gen{c->pts[3 * i], c->pts[3 * i + 1], c->pts[3 * i + 2], dt.leaf_end[s]};
       std::memcpy(&fslot[s * c->nf], &c->fields[i * c->nf], c->nf * sizeof(double));
     }
     c->free_device();
     c->tree_built = false;
     HIP_OK(hipGetDevice(&c->device));
-    c->d_nodes = dev_alloc<Node>(t.nodes.size());
-    // kLeafMaxSize zero records of padding: leaf scans load whole leaves.
+    c->d_nodes = dev_alloc<DNode>(std::max<size_t>(1, dt.nodes.size()));
+    // kLeafMaxSize zero records of padding: leaf steps load whole chunks.
     c->d_gen = dev_alloc<Gen>(c->n + kLeafMaxSize);
     c->d_fields = dev_alloc<double>(fslot.size());
     c->d_slot_id = dev_alloc<uint32_t>(c->n);
     c->d_err = dev_alloc<ErrRec>(1);
     c->d_stats = dev_alloc<Stats>(1);
-    HIP_OK(hipMemcpy(c->d_nodes, t.nodes.data(), t.nodes.size() * sizeof(Node),
-                     hipMemcpyHostToDevice));
-    gen.resize(c->n + kLeafMaxSize, Gen{0.0, 0.0, 0.0, 0.0});
+    if (!dt.nodes.empty())
+      HIP_OK(hipMemcpy(c->d_nodes, dt.nodes.data(), dt.nodes.size() * sizeof(DNode),
+                       hipMemcpyHostToDevice));
+    gen.resize(c->n + kLeafMaxSize, Gen{0.0, 0.0, 0.0, 0});
     HIP_OK(hipMemcpy(c->d_gen, gen.data(), gen.size() * sizeof(Gen), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_fields, fslot.data(), std::max<size_t>(1, fslot.size()) * sizeof(double),
                      hipMemcpyHostToDevice));
@@ -837,6 +969,7 @@ int vt_cloud_upload(vt_cloud *c) {
     d.n = (uint32_t)c->n;
     d.nf = (uint32_t)c->nf;
     d.periodic = c->periodic;
+    d.root = dt.root;
     c->tree_built = true;
     return VT_OK;
   });
@@ -963,25 +1096,19 @@ int vt_project(const vt_cloud *cc, const double *d_start, const double *d_end, s
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (nrays == 0) return VT_OK;
     const int nout = mode == VT_VOLUME ? 3 : (int)c->nf;
-    reset_call(c, st);
-    Timer tm;
     const int64_t n = (int64_t)nrays;
     if (mode == VT_SUM) {
       switch (c->nf) {
-        case 1: launch_project<VT_SUM, 1>(c, d_start, d_end, n, d_out, nout, st, tm); break;
-        case 2: launch_project<VT_SUM, 2>(c, d_start, d_end, n, d_out, nout, st, tm); break;
-        case 3: launch_project<VT_SUM, 3>(c, d_start, d_end, n, d_out, nout, st, tm); break;
-        case 4: launch_project<VT_SUM, 4>(c, d_start, d_end, n, d_out, nout, st, tm); break;
-        default: launch_project<VT_SUM, 0>(c, d_start, d_end, n, d_out, nout, st, tm); break;
+        case 1: return project<VT_SUM, 1>(c, d_start, d_end, n, d_out, nout, st);
+        case 2: return project<VT_SUM, 2>(c, d_start, d_end, n, d_out, nout, st);
+        case 3: return project<VT_SUM, 3>(c, d_start, d_end, n, d_out, nout, st);
+        case 4: return project<VT_SUM, 4>(c, d_start, d_end, n, d_out, nout, st);
+        default: return project<VT_SUM, 0>(c, d_start, d_end, n, d_out, nout, st);
       }
-    } else if (mode == VT_MAX) {
-      launch_project<VT_MAX, 0>(c, d_start, d_end, n, d_out, nout, st, tm);
-    } else if (mode == VT_MIN) {
-      launch_project<VT_MIN, 0>(c, d_start, d_end, n, d_out, nout, st, tm);
-    } else {
-      launch_project<VT_VOLUME, 4>(c, d_start, d_end, n, d_out, nout, st, tm);
     }
-    return finish_call(c, st, tm.e0, tm.e1);
+    if (mode == VT_MAX) return project<VT_MAX, 0>(c, d_start, d_end, n, d_out, nout, st);
+    if (mode == VT_MIN) return project<VT_MIN, 0>(c, d_start, d_end, n, d_out, nout, st);
+    return project<VT_VOLUME, 4>(c, d_start, d_end, n, d_out, nout, st);
   });
 }
 
@@ -993,17 +1120,21 @@ int vt_project_count(const vt_cloud *cc, const double *d_start, const double *d_
     vt_cloud *c = const_cast<vt_cloud *>(cc);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (nrays == 0) return VT_OK;
-    reset_call(c, st);
-    Timer tm;
-    auto kc = c->periodic ? k_count<true> : k_count<false>;
-    const int grid = grid_for(kc, (int64_t)nrays, c->device);
-    ensure_scratch(c, (size_t)grid * kBlock);
-    HIP_OK(hipEventRecord(tm.e0, st));
-    kc<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_start, d_end, (int64_t)nrays, d_counts,
-                                c->d_err, c->d_stats);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipEventRecord(tm.e1, st));
-    return finish_call(c, st, tm.e0, tm.e1);
+    const int64_t n = (int64_t)nrays;
+    auto go = [&](auto kfast, auto kslow) {
+      return two_pass(
+          c, st, nrays,
+          [&](Timer &tm) {
+            launch(c, kfast, n, st, tm, d_start, d_end, n, (const uint32_t *)nullptr,
+                   c->d_defer, d_counts, c->d_err, c->d_stats);
+          },
+          [&](Timer &tm, int64_t nd) {
+            launch(c, kslow, nd, st, tm, d_start, d_end, nd, (const uint32_t *)c->d_defer,
+                   (uint32_t *)nullptr, d_counts, c->d_err, c->d_stats);
+          });
+    };
+    return c->periodic ? go(k_count<true, false>, k_count<true, true>)
+                       : go(k_count<false, false>, k_count<false, true>);
   });
 }
 
@@ -1019,31 +1150,31 @@ int vt_project_segments(const vt_cloud *cc, const double *d_start, const double 
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (nrays == 0) return VT_OK;
     const int nout = mode == VT_VOLUME ? 3 : (int)c->nf;
-    reset_call(c, st);
-    Timer tm;
     const int64_t n = (int64_t)nrays;
-    auto go = [&](auto kernel) {
-      const int grid = grid_for(kernel, n, c->device);
-      ensure_scratch(c, (size_t)grid * kBlock);
-      HIP_OK(hipEventRecord(tm.e0, st));
-      kernel<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, d_start, d_end, n, d_offsets, d_out,
-                                      nout, c->d_slot_id, d_cell_ids, d_s_enter, d_s_exit,
-                                      c->d_err, c->d_stats);
-      HIP_OK(hipGetLastError());
-      HIP_OK(hipEventRecord(tm.e1, st));
+    auto go = [&](auto kfast, auto kslow) {
+      return two_pass(
+          c, st, nrays,
+          [&](Timer &tm) {
+            launch(c, kfast, n, st, tm, d_start, d_end, n, (const uint32_t *)nullptr,
+                   c->d_defer, d_offsets, d_out, nout, (const uint32_t *)c->d_slot_id,
+                   d_cell_ids, d_s_enter, d_s_exit, c->d_err, c->d_stats);
+          },
+          [&](Timer &tm, int64_t nd) {
+            launch(c, kslow, nd, st, tm, d_start, d_end, nd, (const uint32_t *)c->d_defer,
+                   (uint32_t *)nullptr, d_offsets, d_out, nout, (const uint32_t *)c->d_slot_id,
+                   d_cell_ids, d_s_enter, d_s_exit, c->d_err, c->d_stats);
+          });
     };
     if (c->periodic) {
-      if (mode == VT_SUM) go(k_fill<VT_SUM, true>);
-      if (mode == VT_MAX) go(k_fill<VT_MAX, true>);
-      if (mode == VT_MIN) go(k_fill<VT_MIN, true>);
-      if (mode == VT_VOLUME) go(k_fill<VT_VOLUME, true>);
-    } else {
-      if (mode == VT_SUM) go(k_fill<VT_SUM, false>);
-      if (mode == VT_MAX) go(k_fill<VT_MAX, false>);
-      if (mode == VT_MIN) go(k_fill<VT_MIN, false>);
-      if (mode == VT_VOLUME) go(k_fill<VT_VOLUME, false>);
+      if (mode == VT_SUM) return go(k_fill<VT_SUM, true, false>, k_fill<VT_SUM, true, true>);
+      if (mode == VT_MAX) return go(k_fill<VT_MAX, true, false>, k_fill<VT_MAX, true, true>);
+      if (mode == VT_MIN) return go(k_fill<VT_MIN, true, false>, k_fill<VT_MIN, true, true>);
+      return go(k_fill<VT_VOLUME, true, false>, k_fill<VT_VOLUME, true, true>);
     }
-    return finish_call(c, st, tm.e0, tm.e1);
+    if (mode == VT_SUM) return go(k_fill<VT_SUM, false, false>, k_fill<VT_SUM, false, true>);
+    if (mode == VT_MAX) return go(k_fill<VT_MAX, false, false>, k_fill<VT_MAX, false, true>);
+    if (mode == VT_MIN) return go(k_fill<VT_MIN, false, false>, k_fill<VT_MIN, false, true>);
+    return go(k_fill<VT_VOLUME, false, false>, k_fill<VT_VOLUME, false, true>);
   });
 }
 

@@ -1,19 +1,19 @@
 // Device ray walk: the reference's split-point traversal (src/ray.cpp:187-430,
-// docs/algorithm.rst, docs/edgecases.rst) for one ray per lane.
+// docs/algorithm.rst, docs/edgecases.rst), one ray per lane.
 //
 // The reference keeps the ray's points in a singly linked list; new points are
 // only ever inserted right after the current one, so the points ahead of the
 // current one form a stack.  Here that stack keeps its top (the "next" point)
 // in registers and the rest in per-lane device memory, interleaved by lane.
 //
-// The walk is written as a state machine with ONE nearest-neighbour call site:
-// every step first advances the lane's state to its next query (endpoint,
-// dyadic probe, bisection probe, split-point classification, along-ray or
-// perpendicular perturbation), then runs the query, then consumes the result.
-// Lanes of a wave that are in different phases of their walks therefore share
-// the same traversal code, and only one copy of it (and of the periodic image
-// merge) is compiled.  Each step makes exactly the queries, in the order and
-// with the arithmetic, of the reference.
+// The walk is a re-entrant state machine around ONE nearest-neighbour query:
+// prepare() advances the lane to its next query (endpoint, dyadic probe,
+// bisection probe, split-point classification, along-ray or perpendicular
+// perturbation) and returns its point and k; consume() takes the query's
+// result and moves the state on.  The caller runs the query in between, so a
+// wave can interleave the walks and kNN traversals of its lanes freely.  Each
+// step makes exactly the queries, in the order and with the arithmetic, of the
+// reference.
 //
 // Segments are handed to a sink as they close, in order, which fuses the
 // reductions of src/reduction.cpp into the walk.
@@ -22,20 +22,21 @@
 
 namespace vt {
 
+// Ray::Ray, src/ray.cpp:14-35: the unit direction and length.  The end point
+// is not kept; the one query at the end reloads it.
 struct RayGeom {
   double a[3];  // start
-  double b[3];  // end
   double d[3];  // unit direction
   double len;
 };
 
-// Ray::Ray, src/ray.cpp:14-35.
 __device__ __forceinline__ void make_ray(const double *s, const double *e, RayGeom &r) {
+  double b[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     r.a[j] = gld(s + j);
-    r.b[j] = gld(e + j);
-    r.d[j] = r.b[j] - r.a[j];
+    b[j] = gld(e + j);
+    r.d[j] = b[j] - r.a[j];
   }
   const double L = sqrt(r.d[0] * r.d[0] + r.d[1] * r.d[1] + r.d[2] * r.d[2]);
   r.d[0] /= L;
@@ -53,13 +54,10 @@ struct LaneInfo {
 };
 
 __device__ __forceinline__ KStack make_stack(const Scratch &S, const LaneInfo &L) {
-  return KStack{L.tid, S.knn_tag + L.gtid, S.knn_val + L.gtid, S.lanes, S.kspill, 0, 0, false, 0};
+  return KStack{L.tid,   S.knn_tag + L.gtid, S.knn_val + L.gtid, S.knn_pb + L.gtid,
+                S.lanes, S.kspill,           0,                  0,
+                false,   0};
 }
-
-struct WalkCounters {
-  uint64_t nq = 0;     // kNN queries
-  uint64_t nslow = 0;  // K_EXPAND searches
-};
 
 // findSplitPointDistance(pos1, pos2), src/ray.cpp:38-60.
 __device__ __forceinline__ double split_analytic(const RayGeom &r, const double *p1,
@@ -111,16 +109,21 @@ __device__ __forceinline__ bool split_point(const DevCloud &C, const RayGeom &r,
 
 // The K_EXPAND re-classification (src/ray.cpp:320-338), in the wave's device
 // buffer; the caller runs the lanes of a wave that need it one at a time.
+// The result is packed as error | has_c << 8 | has_n << 9.
 template <bool kPer>
-__device__ int classify_expand(const DevCloud &C, const Scratch &S, const LaneInfo &L,
-                               KStack &st, const double *pos, uint32_t ct, uint32_t nt,
-                               bool &has_c, bool &has_n) {
+__device__ __forceinline__ int classify_expand(const DevCloud &C, const Scratch &S, int tid,
+                                            int gtid, double x, double y, double z,
+                                            uint32_t ct, uint32_t nt) {
+  KStack st{tid, S.knn_tag + gtid, S.knn_val + gtid, S.knn_pb + gtid, S.lanes, S.kspill,
+            0,   0,                false, 0};
+  const int wave = gtid >> 6;
   const int kb = kExpandK < (int)C.n ? kExpandK : (int)C.n;
-  uint32_t *bi = S.big_id + (size_t)L.wave * 4 * kMaxQueryK;
-  double *bd = S.big_d + (size_t)L.wave * 4 * kMaxQueryK;
+  uint32_t *bi = S.big_id + (size_t)wave * 4 * kMaxQueryK;
+  double *bd = S.big_d + (size_t)wave * 4 * kMaxQueryK;
   MemRes res{bd, bi, kb, 0};
-  query_mem<kPer>(C, pos[0], pos[1], pos[2], kb, res, bi + 2 * kMaxQueryK, bd + 2 * kMaxQueryK,
-                  st);
+  query_mem<kPer>(C, x, y, z, kb, res, bi + 2 * kMaxQueryK, bd + 2 * kMaxQueryK, st);
+  if (st.overflow) return kErrKnnStack;
+  bool has_c = false, has_n = false;
   const double d0 = gld(bd);
   for (int i = 0; i < kb; ++i) {
     if (gld(bd + i) - d0 >= kTol) break;
@@ -134,7 +137,7 @@ __device__ int classify_expand(const DevCloud &C, const Scratch &S, const LaneIn
       if (kb == kExpandK) return kErrTies;
     }
   }
-  return kErrNone;
+  return (has_c ? 1 << 8 : 0) | (has_n ? 1 << 9 : 0);
 }
 
 // Walk states.  A state names the query the lane issues next.
@@ -149,34 +152,54 @@ enum : int {
   kStAlongN,     // 2-NN at pos + eps * dir (case 1a, n side)
   kStPerpC,      // 1-NN of the perpendicular cycling, c side
   kStPerpN,      // 1-NN of the perpendicular cycling, n side
-  kStDone,
+  kStFinal,      // (no query) the last segment [entry_s, sc] is still open
+  kStDone,       // every segment has been emitted
 };
 
-// Walks one ray; emits its segments to `sink` in order.  Returns an error code
-// (kErrNone on success) and fills ev / ei with the values the host formats
-// into the reference's message.
-template <bool kPer, class Sink>
-__device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, KStack &st,
-                        const RayGeom &r, Sink &sink, int &max_depth, WalkCounters &cnt,
-                        double *ev, uint32_t *ei) {
-  const size_t wbase = (size_t)L.gtid;
-  // current point, and the top of the pending stack ("next")
-  uint32_t ct = 0, nt = 0;
-  double sc = 0.0, sn = r.len;
-  int wsp = 0;  // pending entries below the top, in device memory
-  uint32_t open_cell = 0;
-  double entry_s = 0.0;
-  int iteration = 0;
-  int state = kStStart;
-  double s = 0.0;          // current split point
-  uint32_t c0 = 0;         // start cell
-  uint32_t side_cell = 0;  // case 1a: the c-side cell once resolved
+// Error details for the host's message (the reference's exception texts).
+struct WalkErr {
+  double v[3];
+  uint32_t id[2];
+};
+
+// One ray's walk.  Per-lane scratch: the pending stack below its top lives in
+// S.walk_tag / S.walk_s at [depth * S.lanes + wbase].
+template <bool kPer>
+struct Walker {
+  RayGeom r;
+  uint32_t ct, nt;  // current point, and the top of the pending stack ("next")
+  double sc, sn;
+  int wsp;  // pending entries below the top, in device memory
+  uint32_t open_cell;
+  double entry_s;
+  int iteration;
+  int state;
+  double s;            // current split point
+  uint32_t c0;         // start cell
+  uint32_t side_cell;  // case 1a: the c-side cell once resolved
   // per-state registers: dyadic (level, i), bisection (it, lo, hi),
   // perpendicular cycling (bound, a, b, c, cycle)
-  int ia = 0, ib = 0, ic = 0, id = 0, ie = 0;
-  double lo = 0.0, hi = 0.0;
+  int ia, ib, ic, id, ie;
+  double lo, hi;
 
-  auto push_top = [&](uint32_t cell, double sv) -> bool {
+  __device__ __forceinline__ void begin(const double *s0, const double *e0) {
+    make_ray(s0, e0, r);
+    ct = nt = 0;
+    sc = 0.0;
+    sn = r.len;
+    wsp = 0;
+    open_cell = 0;
+    entry_s = 0.0;
+    iteration = 0;
+    state = kStStart;
+    s = 0.0;
+    c0 = side_cell = 0;
+    ia = ib = ic = id = ie = 0;
+    lo = hi = 0.0;
+  }
+
+  __device__ __forceinline__ bool push_top(const Scratch &S, size_t wbase, int &max_depth,
+                                           uint32_t cell, double sv) {
     if (wsp >= S.wcap) return false;
     gst(S.walk_tag + (size_t)wsp * S.lanes + wbase, nt);
     gst(S.walk_s + (size_t)wsp * S.lanes + wbase, sn);
@@ -185,9 +208,9 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
     nt = cell;
     sn = sv;
     return true;
-  };
+  }
   // Advance the current point to the next one; false when none is left.
-  auto advance = [&]() -> bool {
+  __device__ __forceinline__ bool advance(const Scratch &S, size_t wbase) {
     ct = nt;
     sc = sn;
     if (wsp == 0) return false;
@@ -195,38 +218,39 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
     nt = gld(S.walk_tag + (size_t)wsp * S.lanes + wbase);
     sn = gld(S.walk_s + (size_t)wsp * S.lanes + wbase);
     return true;
-  };
+  }
   // The split point was found (analytically or by bisection): bounds check,
   // clamp (src/ray.cpp:280-291), then classify.
-  auto accept_split = [&](double sv) -> int {
+  __device__ __forceinline__ int accept_split(double sv, WalkErr &e) {
     if (sv < sc - kTol || sv > sn + kTol) {
-      ev[0] = sv;
-      ev[1] = sc;
-      ev[2] = sn;
-      ei[0] = ct;
-      ei[1] = nt;
+      e.v[0] = sv;
+      e.v[1] = sc;
+      e.v[2] = sn;
+      e.id[0] = ct;
+      e.id[1] = nt;
       return kErrSplitOutside;
     }
     const double l2 = sc + 1e-15, h2 = sn - 1e-15;
     s = (sv < l2) ? l2 : ((h2 < sv) ? h2 : sv);  // std::clamp
     state = kStClassify;
     return kErrNone;
-  };
+  }
   // Case 1a resolved on both sides: insert one or two points at s.
-  auto resolve_1a = [&](uint32_t scell, uint32_t ncell) -> int {
+  __device__ __forceinline__ int resolve_1a(const Scratch &S, size_t wbase, int &max_depth,
+                                            uint32_t scell, uint32_t ncell) {
     if (scell == ncell) {
-      if (!push_top(scell, s)) return kErrWalkStack;
+      if (!push_top(S, wbase, max_depth, scell, s)) return kErrWalkStack;
     } else {  // current -> scell -> ncell -> next, both at s
-      if (!push_top(ncell, s)) return kErrWalkStack;
-      if (!push_top(scell, s)) return kErrWalkStack;
+      if (!push_top(S, wbase, max_depth, ncell, s)) return kErrWalkStack;
+      if (!push_top(S, wbase, max_depth, scell, s)) return kErrWalkStack;
     }
     state = kStSplit;
     return kErrNone;
-  };
+  }
   // Perpendicular cycling (src/ray.cpp:129-185): move to the next candidate
   // direction that needs a query, counting degenerate ones as cycles.
   // Returns false when MAX_PERTURBATION_CYCLES is reached.
-  auto perp_next = [&](double (&u)[3]) -> bool {
+  __device__ __forceinline__ bool perp_next(const DevCloud &C, double (&u)[3]) {
     const double s2 = C.inv_sqrt2;
     for (;;) {
       if (ie >= kMaxCycles) return false;
@@ -260,33 +284,35 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
       for (int j = 0; j < 3; ++j) u[j] /= mag;
       return true;
     }
-  };
-  auto perp_start = [&](int st_next) {
+  }
+  __device__ __forceinline__ void perp_start(int st_next) {
     ia = 1;   // bound
     ib = -1;  // a
     ic = -1;  // b
     id = -2;  // c (the first perp_next moves it to -1)
     ie = 0;   // cycle
     state = st_next;
-  };
+  }
 
-  for (;;) {
-    // ---- 1. advance to the next query ----
+  // Advances to the next query: on success with state < kStFinal, q and k are
+  // the query; with kStFinal the ray has no query left.
+  __device__ __forceinline__ int prepare(const DevCloud &C, const Scratch &S, size_t wbase, const double *end,
+                         double (&q)[3], int &k, WalkErr &e) {
     if (state == kStSplit) {
       for (;;) {
         if (++iteration > kMaxIter) return kErrMaxIter;
         if (sn - sc < 1e-14) {  // zero-width step
-          const bool more = advance();
+          const bool more = advance(S, wbase);
           open_cell = ct;
           if (!more) {
-            state = kStDone;
-            break;
+            state = kStFinal;
+            return kErrNone;
           }
           continue;
         }
         double sv;
         if (split_point<kPer>(C, r, ct, nt, sc, sn, sv)) {
-          const int rc = accept_split(sv);
+          const int rc = accept_split(sv, e);
           if (rc) return rc;
         } else {
           lo = sc;
@@ -296,19 +322,14 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
         }
         break;
       }
-      if (state == kStDone) break;
     }
-    double pos[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) pos[j] = r.a[j] + s * r.d[j];
-    double q[3];
-    int k = 1;
+    k = 1;
     if (state == kStStart) {
 #pragma unroll
       for (int j = 0; j < 3; ++j) q[j] = r.a[j];
     } else if (state == kStEnd) {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) q[j] = r.b[j];
+      for (int j = 0; j < 3; ++j) q[j] = gld(end + j);
     } else if (state == kStDyadic) {
       const double mid_s = r.len * ib / (1 << ia);
 #pragma unroll
@@ -317,36 +338,48 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
       const double mid_s = 0.5 * (lo + hi);
 #pragma unroll
       for (int j = 0; j < 3; ++j) q[j] = r.a[j] + mid_s * r.d[j];
-    } else if (state == kStClassify) {
+    } else {
+      double pos[3];
 #pragma unroll
-      for (int j = 0; j < 3; ++j) q[j] = pos[j];
-      k = 3;
-    } else if (state == kStAlongC || state == kStAlongN) {
-      const double sign = state == kStAlongC ? -1.0 : 1.0;
+      for (int j = 0; j < 3; ++j) pos[j] = r.a[j] + s * r.d[j];
+      if (state == kStClassify) {
 #pragma unroll
-      for (int j = 0; j < 3; ++j) q[j] = pos[j] + sign * kPertEps * r.d[j];
-      k = 2;
-    } else {  // kStPerpC / kStPerpN: around pos -/+ eps * dir
-      double u[3];
-      if (!perp_next(u)) return kErrPerturb;
-      const bool c_side = state == kStPerpC;
+        for (int j = 0; j < 3; ++j) q[j] = pos[j];
+        k = 3;
+      } else if (state == kStAlongC || state == kStAlongN) {
+        const double sign = state == kStAlongC ? -1.0 : 1.0;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const double base = c_side ? pos[j] - kPertEps * r.d[j] : pos[j] + kPertEps * r.d[j];
-        q[j] = base + kPertEps * u[j];
+        for (int j = 0; j < 3; ++j) q[j] = pos[j] + sign * kPertEps * r.d[j];
+        k = 2;
+      } else {  // kStPerpC / kStPerpN: around pos -/+ eps * dir
+        double u[3];
+        if (!perp_next(C, u)) return kErrPerturb;
+        const bool c_side = state == kStPerpC;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const double base = c_side ? pos[j] - kPertEps * r.d[j] : pos[j] + kPertEps * r.d[j];
+          q[j] = base + kPertEps * u[j];
+        }
       }
     }
+    return kErrNone;
+  }
 
-    // ---- 2. the query ----
-    uint32_t ids[3];
-    double r2[3];
-    query_rt<kPer>(C, q[0], q[1], q[2], k, ids, r2, st);
-    ++cnt.nq;
+  // Takes the result of the query prepare() asked for.  May emit segments; on
+  // return the state is the next query's, kStSplit, kStFinal or kStDone.
+  // Without kExpand, a ray that needs the K_EXPAND classification returns
+  // kDeferRay.  With it, `nslow` counts K_EXPAND searches, and every lane of
+  // the wave that calls consume() must do so together (K_EXPAND runs those
+  // lanes one at a time).
+  template <bool kExpand, class Sink>
+  __device__ __forceinline__ int consume(const DevCloud &C, const Scratch &S, size_t wbase,
+                                         int tid, int gtid, const uint32_t (&ids)[3],
+                                         const double (&r2)[3], Sink &sink, int &max_depth,
+                                         uint64_t &nslow, WalkErr &e) {
     // Ids index device arrays: a search that inserted nothing (NaN or
     // overflowing coordinates) fails the ray instead.
     if (ids[0] == kNone) return kErrNonFinite;
-
-    // ---- 3. consume the result ----
+    const int wlane = tid & 63;
     if (state == kStStart) {
       c0 = ids[0];
       state = kStEnd;
@@ -359,7 +392,7 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
         state = kStSplit;
       } else if constexpr (!kPer) {
         sink.seg(c0, 0.0, r.len);
-        return kErrNone;
+        state = kStDone;
       } else {
         ia = 1;  // level
         ib = 1;  // i
@@ -368,7 +401,7 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
     } else if (state == kStDyadic) {
       if (ids[0] != c0) {
         const double mid_s = r.len * ib / (1 << ia);
-        if (!push_top(ids[0], mid_s)) return kErrWalkStack;
+        if (!push_top(S, wbase, max_depth, ids[0], mid_s)) return kErrWalkStack;
         state = kStSplit;
       } else {
         ib += 2;
@@ -383,17 +416,17 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
       const uint32_t mc = ids[0];
       int rc = kErrNone;
       if (mc != ct && mc != nt) {
-        rc = accept_split(mid_s);
+        rc = accept_split(mid_s, e);
       } else {
         if (mc == ct)
           lo = mid_s;
         else
           hi = mid_s;
         if (hi - lo < 1e-12) {
-          rc = accept_split(0.5 * (lo + hi));
+          rc = accept_split(0.5 * (lo + hi), e);
         } else if (++ia >= 64) {
-          ei[0] = ct;
-          ei[1] = nt;
+          e.id[0] = ct;
+          e.id[1] = nt;
           return kErrBisect;
         }
       }
@@ -412,29 +445,33 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
         }
       }
       if (three_eq && !has_c && !has_n) {
-        int rc = kErrNone;
+        if constexpr (!kExpand) return kDeferRay;
+        double pos[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) pos[j] = r.a[j] + s * r.d[j];
+        int res = 0;
         uint64_t want = __ballot(1);
         while (want) {
           const int l = __ffsll((unsigned long long)want) - 1;
-          if (l == L.lane) {
-            rc = classify_expand<kPer>(C, S, L, st, pos, ct, nt, has_c, has_n);
-            ++cnt.nq;
-            ++cnt.nslow;
+          if (l == wlane) {
+            res = classify_expand<kPer>(C, S, tid, gtid, pos[0], pos[1], pos[2], ct, nt);
+            ++nslow;
           }
           want &= want - 1;
         }
-        if (rc) return rc;
+        if (res & 0xff) return res & 0xff;
+        has_c = (res >> 8) & 1;
+        has_n = (res >> 9) & 1;
       }
       if (has_c || has_n) {
         // Case 2: the c/n boundary.
         sink.seg(open_cell, entry_s, s);
         open_cell = nt;
         entry_s = s;
-        if (!advance()) break;
-        state = kStSplit;
+        state = advance(S, wbase) ? kStSplit : kStFinal;
       } else if (r2[0] + kTol < r2[1]) {
         // Case 1: one cell strictly closer; insert it.
-        if (!push_top(ids[0], s)) return kErrWalkStack;
+        if (!push_top(S, wbase, max_depth, ids[0], s)) return kErrWalkStack;
         state = kStSplit;
       } else {
         // Case 1a: resolve each side by perturbation along, then across, the ray.
@@ -450,7 +487,7 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
           perp_start(kStPerpC);
         }
       } else if (ok) {
-        const int rc = resolve_1a(side_cell, ids[0]);
+        const int rc = resolve_1a(S, wbase, max_depth, side_cell, ids[0]);
         if (rc) return rc;
       } else {
         perp_start(kStPerpN);
@@ -462,16 +499,15 @@ __device__ int walk_ray(const DevCloud &C, const Scratch &S, const LaneInfo &L, 
           side_cell = cell;
           state = kStAlongN;
         } else {
-          const int rc = resolve_1a(side_cell, cell);
+          const int rc = resolve_1a(S, wbase, max_depth, side_cell, cell);
           if (rc) return rc;
         }
       } else {
         ++ie;  // cycle
       }
     }
+    return kErrNone;
   }
-  sink.seg(open_cell, entry_s, sc);
-  return kErrNone;
-}
+};
 
 }  // namespace vt
