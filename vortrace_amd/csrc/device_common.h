@@ -63,9 +63,17 @@ struct Stats {
   unsigned int max_walk_depth;
   unsigned int max_knn_depth;
   unsigned long long slow_queries;
-  unsigned long long ray_counter;  // work queue head
+  unsigned long long ray_counter;  // work queue head (query kernels)
   unsigned long long deferred;     // rays appended to the deferred list
+  unsigned long long part_counter[8];  // ray engine: work queue head per partition
 };
+
+// The ray engine splits its rays into kParts contiguous partitions, one per
+// XCD: workgroups are dispatched to the XCDs round-robin, so workgroup b runs
+// on XCD b mod 8 and its lanes take rays from partition b mod 8 first.  Rays
+// of one partition cover one band of the image, which keeps each XCD's L2
+// working set to the cells under that band.
+constexpr int kParts = 8;
 
 // Everything a kernel needs to know about a built cloud (passed by value).
 struct DevCloud {
@@ -83,9 +91,7 @@ struct DevCloud {
 
 // Per-lane scratch that does not fit in registers/LDS (device memory).
 struct Scratch {
-  uint32_t *knn_tag;   // [kspill][T] kNN stack entries beyond the LDS part
-  double *knn_val;
-  float *knn_pb;
+  uint4 *knn_spill;    // [kspill][T] kNN stack entries beyond the LDS part
   uint32_t *walk_tag;  // [wcap][T] pending ray points below the top
   double *walk_s;
   uint32_t *big_id;    // [T][3 * kMaxQueryK] k > 3 result / merge buffers
@@ -94,7 +100,7 @@ struct Scratch {
   int lanes;           // T
   int walk_batch;      // ray engine: lanes of a wave that run their walk steps together
   int leaf_batch;      // ray engine: lanes of a wave that run their leaf steps together
-  int pad;
+  int parts;           // ray engine: ray partitions (1 or kParts)
 };
 
 constexpr int kMaxQueryK = 128;
@@ -115,6 +121,21 @@ typedef double v2f64 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ double2 gld(const double2 *p) {
   const v2f64 v = *(const __attribute__((address_space(1))) v2f64 *)(p);
   return make_double2(v.x, v.y);
+}
+
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+// 16-byte global load / store of four dwords.
+__device__ __forceinline__ uint4 gld(const uint4 *p) {
+  const v4u32 v = *(const __attribute__((address_space(1))) v4u32 *)(p);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst(uint4 *p, uint4 e) {
+  v4u32 v;
+  v.x = e.x;
+  v.y = e.y;
+  v.z = e.z;
+  v.w = e.w;
+  *(__attribute__((address_space(1))) v4u32 *)(p) = v;
 }
 
 // a, b or c for f = 0, 1 or 2 (by value).

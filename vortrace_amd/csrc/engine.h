@@ -80,8 +80,7 @@ struct Search {
         mind += dd[i];
       }
     }
-    st.sp = 0;
-    st.base = 0;
+    st.reset();
     ptop = INFINITY;
     cur = C.root;
     mode = (cur & kRefLeaf) ? kTmLeaf : kTmNode;
@@ -132,14 +131,15 @@ struct Search {
     if (depth >= 0) {
       // First descent (worst distance still infinite): every far child goes
       // into the LDS ring at slot depth mod kLdsDepth.
-      const int sl = (depth & (kLdsDepth - 1)) * kBlock + st.tid;
+      const int sl = st.ring_slot(depth);
       if (depth >= kLdsDepth) {
-        const double old = lds_stack_val[sl];
+        uint32_t ot;
+        double old;
+        float op;
+        unpack_entry(lds_entry(sl), ot, old, op);
         lost_min = old < lost_min ? old : lost_min;
       }
-      lds_stack_tag[sl] = tag;
-      lds_stack_val[sl] = m2;
-      lds_stack_pb[sl] = ptop;
+      set_lds_entry(sl, pack_entry(tag, m2, ptop));
       ptop = fminf(ptop, bound_below(m2));
       ++depth;
     } else if (m2 <= rs.worst()) {
@@ -157,35 +157,51 @@ struct Search {
   }
 
   // Takes entries off the stack until one is visited (mode kTmEnter or
-  // kTmLeaf) or nothing left can be (kTmDone).
+  // kTmLeaf) or nothing left can be (kTmDone).  The top kPopBatch entries are
+  // read at once and resolved in registers, top first.
+  static constexpr int kPopBatch = 4;
   __device__ __forceinline__ void pop(KStack &st) {
     const double worst = rs.worst();
-    for (;;) {
-      if (st.sp == 0) break;
-      uint32_t tag;
-      double v;
-      float pb;
-      st.pop(tag, v, pb);
-      ptop = pb;
-      if (tag == kSkip) continue;
-      if ((tag & kUndo) == kUndo) {
-        set_dd(tag & 3u, v);
-      } else if (v <= worst) {  // mindistsq * epsError <= worstDist()
-        if (tag & kRefLeaf) {
-          cur = tag;
-          mode = kTmLeaf;
-        } else {
-          cur = tag & kIdxMask;
-          pdim = (tag >> 29) & 3u;
-          pv = v;
-          mode = kTmEnter;
+    while (st.sp > 0) {
+      uint4 e[kPopBatch];
+#pragma unroll
+      for (int j = 0; j < kPopBatch; ++j)
+        if (j < st.sp) e[j] = st.read(st.sp - 1 - j);
+      int used = 0;
+      bool found = false, stop = false;
+#pragma unroll
+      for (int j = 0; j < kPopBatch; ++j) {
+        if (found || stop || j >= st.sp) continue;
+        used = j + 1;
+        uint32_t tag;
+        double v;
+        float pb;
+        unpack_entry(e[j], tag, v, pb);
+        ptop = pb;
+        if (tag == kSkip) continue;
+        if ((tag & kUndo) == kUndo) {
+          set_dd(tag & 3u, v);
+        } else if (v <= worst) {  // mindistsq * epsError <= worstDist()
+          if (tag & kRefLeaf) {
+            cur = tag;
+            mode = kTmLeaf;
+          } else {
+            cur = tag & kIdxMask;
+            pdim = (tag >> 29) & 3u;
+            pv = v;
+            mode = kTmEnter;
+          }
+          found = true;
+          continue;
         }
-        return;
+        // Nothing below can be visited any more: nanoflann prunes all of it.
+        if ((double)pb > worst) stop = true;
       }
-      // Nothing below can be visited any more: nanoflann prunes all of it.
-      if ((double)pb > worst) break;
+      st.drop(used);
+      if (found) return;
+      if (stop) break;
     }
-    st.sp = 0;
+    st.reset();
     mode = kTmDone;
   }
 
@@ -233,8 +249,7 @@ struct Search {
         } else {
           // An entry that fell out of the ring can still be visited: walk
           // the same path again, pushing the far children that can.
-          st.sp = 0;
-          st.base = 0;
+          st.reset();
           ptop = INFINITY;
           cur = C.root;
           mode = kTmNode;
@@ -383,6 +398,10 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
   int64_t ray = 0;
   const int batch = S.walk_batch;
   const int leaf_batch = S.leaf_batch;
+  const int nparts = S.parts;
+  const int64_t psize = (nrays + nparts - 1) / nparts;
+  int part = blockIdx.x & (nparts - 1);
+  int part_tries = 0;
   for (;;) {
     {  // walk steps of every waiting lane
       WalkErr e{{0.0, 0.0, 0.0}, {0u, 0u}};
@@ -419,8 +438,19 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
         }
       }
       if (ph == kPhFetch) {
-        ray = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
-        if (ray >= nrays ||
+        // The lane's own partition first, then the others in turn.
+        ray = -1;
+        while (part_tries < nparts) {
+          const int64_t lo = part * psize;
+          const int64_t idx = (int64_t)atomicAdd(&stats->part_counter[part], 1ULL);
+          if (idx < psize && lo + idx < nrays) {
+            ray = lo + idx;
+            break;
+          }
+          part = (part + 1) & (nparts - 1);
+          ++part_tries;
+        }
+        if (ray < 0 ||
             __hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           ph = kPhDone;
         } else {

@@ -38,68 +38,83 @@ constexpr uint32_t kSkip = 0xFFFFFFFFu;
 constexpr int kBlock = 128;
 constexpr int kLdsDepth = 16;
 
-// The LDS part of every lane's kNN stack: entry i of lane t at [i * kBlock + t],
-// so lanes at equal depth touch consecutive banks.
-__shared__ uint32_t lds_stack_tag[kLdsDepth * kBlock];
-__shared__ double lds_stack_val[kLdsDepth * kBlock];
-__shared__ float lds_stack_pb[kLdsDepth * kBlock];
+// The LDS part of every lane's kNN stack: kLdsDepth 16-byte entries per lane,
+// lane t's at [t * kLdsStride, t * kLdsStride + kLdsDepth).  The stride of 17
+// entries (68 dwords) starts the 16 lanes of a quarter-wave on 16 different
+// 4-bank groups, so 16-byte reads and writes at equal depth do not conflict.
+constexpr int kLdsStride = kLdsDepth + 1;
+__shared__ uint4 lds_stack[kBlock * kLdsStride];
 
-// Per-lane explicit stack.  Logical entries [0, kLdsDepth) live in LDS at slot
-// (base + i) mod kLdsDepth (base is non-zero only after a first descent that
-// filled the LDS part as a ring); deeper entries spill to device memory,
-// interleaved by lane so lanes at equal depth coalesce.  Each entry holds a
-// tag, a value and `pb`, the lower bound of the box distances below it.
+// A stack entry: tag, the lower bound `pb` of the box distances below it, and
+// a value (a box distance, or a saved dists[dim] for undo entries).
+__device__ __forceinline__ uint4 pack_entry(uint32_t tag, double v, float pb) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  return make_uint4(tag, __float_as_uint(pb), (uint32_t)b, (uint32_t)(b >> 32));
+}
+__device__ __forceinline__ void unpack_entry(const uint4 e, uint32_t &tag, double &v, float &pb) {
+  tag = e.x;
+  pb = __uint_as_float(e.y);
+  v = __longlong_as_double((long long)(((uint64_t)e.w << 32) | e.z));
+}
+__device__ __forceinline__ uint4 lds_entry(int i) { return lds_stack[i]; }
+__device__ __forceinline__ void set_lds_entry(int i, uint4 e) { lds_stack[i] = e; }
+
+// Per-lane explicit stack.  The top kLdsDepth entries live in LDS: logical
+// entry i of the window [lo, sp) at slot (base + i) mod kLdsDepth (base is
+// non-zero only after a first descent that filled the LDS part as a ring).
+// A push into a full window first moves the window's bottom entry to device
+// memory (entry i at gs[i * lanes], interleaved by lane so lanes at equal
+// depth coalesce); pops below the window read it back from there.
 static_assert((kLdsDepth & (kLdsDepth - 1)) == 0, "kLdsDepth must be a power of two");
 
 struct KStack {
-  int tid;       // lane within the workgroup
-  uint32_t *gt;  // this lane's spill: entry i >= kLdsDepth at gt[(i - kLdsDepth) * lanes]
-  double *gv;
-  float *gp;
+  int tid;    // lane within the workgroup
+  uint4 *gs;  // this lane's entries below the window
   int lanes, gcap;
   int sp, maxsp;
   bool overflow;
   int base;
+  int lo;  // bottom of the LDS window
 
   __device__ __forceinline__ int slot(int i) const {
-    return ((base + i) & (kLdsDepth - 1)) * kBlock + tid;
+    return tid * kLdsStride + ((base + i) & (kLdsDepth - 1));
+  }
+  // Ring slot of depth d on a first descent (base 0).
+  __device__ __forceinline__ int ring_slot(int d) const {
+    return tid * kLdsStride + (d & (kLdsDepth - 1));
+  }
+  __device__ __forceinline__ void reset() {
+    sp = 0;
+    base = 0;
+    lo = 0;
   }
   __device__ __forceinline__ void push(uint32_t tag, double v, float pb) {
-    if (sp < kLdsDepth) {
-      const int sl = slot(sp);
-      lds_stack_tag[sl] = tag;
-      lds_stack_val[sl] = v;
-      lds_stack_pb[sl] = pb;
-    } else if (sp - kLdsDepth < gcap) {
-      const size_t o = (size_t)(sp - kLdsDepth) * lanes;
-      gst(gt + o, tag);
-      gst(gv + o, v);
-      gst(gp + o, pb);
-    } else {
-      overflow = true;
+    if (sp - lo >= kLdsDepth) {
+      if (lo < gcap)
+        gst(gs + (size_t)lo * lanes, lds_entry(slot(lo)));
+      else
+        overflow = true;
+      ++lo;
     }
+    set_lds_entry(slot(sp), pack_entry(tag, v, pb));
     ++sp;
     maxsp = sp > maxsp ? sp : maxsp;
   }
-  // An entry lost to overflow reads as (kSkip, 0, 0): skipped, and its bound
-  // of 0 stops nothing.
+  // Entry i (i < sp).  An entry lost to overflow reads as (kSkip, 0, 0):
+  // skipped, and its bound of 0 stops nothing.
+  __device__ __forceinline__ uint4 read(int i) const {
+    if (i >= lo) return lds_entry(slot(i));
+    if (i < gcap) return gld(gs + (size_t)i * lanes);
+    return make_uint4(kSkip, 0u, 0u, 0u);
+  }
+  // Drops the top n entries.
+  __device__ __forceinline__ void drop(int n) {
+    sp -= n;
+    lo = lo < sp ? lo : sp;
+  }
   __device__ __forceinline__ void pop(uint32_t &tag, double &v, float &pb) {
-    --sp;
-    if (sp < kLdsDepth) {
-      const int sl = slot(sp);
-      tag = lds_stack_tag[sl];
-      v = lds_stack_val[sl];
-      pb = lds_stack_pb[sl];
-    } else if (sp - kLdsDepth < gcap) {
-      const size_t o = (size_t)(sp - kLdsDepth) * lanes;
-      tag = gld(gt + o);
-      v = gld(gv + o);
-      pb = gld(gp + o);
-    } else {
-      tag = kSkip;
-      v = 0.0;
-      pb = 0.0f;
-    }
+    unpack_entry(read(sp - 1), tag, v, pb);
+    drop(1);
   }
 };
 
@@ -282,8 +297,7 @@ __device__ void knn_core(const DevCloud &C, const double q0, const double q1, co
     }
   }
   // The stack is empty between queries.
-  st.sp = 0;
-  st.base = 0;
+  st.reset();
   float ptop = INFINITY;  // bound of everything on the stack
 
   // Phase 1: best-child descent to the first leaf.  The far child of every
@@ -297,14 +311,15 @@ __device__ void knn_core(const DevCloud &C, const double q0, const double q1, co
     double cut;
     const bool low_first = choose(nd, pick(q0, q1, q2, nd.dim), cut);
     const double m2 = mind + cut - sel3(dd[0], dd[1], dd[2], nd.dim);
-    const int sl = (depth & (kLdsDepth - 1)) * kBlock + st.tid;
+    const int sl = st.ring_slot(depth);
     if (depth >= kLdsDepth) {
-      const double old = lds_stack_val[sl];
+      uint32_t ot;
+      double old;
+      float op;
+      unpack_entry(lds_entry(sl), ot, old, op);
       lost_min = old < lost_min ? old : lost_min;
     }
-    lds_stack_tag[sl] = far_tag(low_first ? nd.c2 : nd.c1, nd.dim);
-    lds_stack_val[sl] = m2;
-    lds_stack_pb[sl] = ptop;
+    set_lds_entry(sl, pack_entry(far_tag(low_first ? nd.c2 : nd.c1, nd.dim), m2, ptop));
     ptop = fminf(ptop, bound_below(m2));
     ++depth;
     ref = low_first ? nd.c1 : nd.c2;
@@ -322,6 +337,7 @@ __device__ void knn_core(const DevCloud &C, const double q0, const double q1, co
   } else {
     // An entry that fell out of the ring can still be visited: walk the
     // same path again and push every far child that can still be visited.
+    st.reset();
     ptop = INFINITY;
     ref = C.root;
     while (ref != first_leaf) {
@@ -386,7 +402,7 @@ __device__ void knn_core(const DevCloud &C, const double q0, const double q1, co
     // Nothing below can be visited any more: nanoflann prunes all of it.
     if ((double)pb > worst) break;
   }
-  st.sp = 0;
+  st.reset();
 }
 
 // PointCloud::minDistSqToBox, src/pointcloud.cpp:283-298.

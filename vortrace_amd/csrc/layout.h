@@ -62,7 +62,8 @@ static_assert(sizeof(DNode) == 32, "device node must be 32 bytes");
 
 struct alignas(16) Gen {  // one generator, slot order
   double x, y, z;
-  uint64_t end;           // end slot of the generator's leaf
+  uint32_t start, end;    // slot range of the generator's leaf (kNone-like
+                          // 0xFFFFFFFF in the padding records after the last slot)
 };
 static_assert(sizeof(Gen) == 32, "generator record must be 32 bytes");
 

@@ -221,12 +221,16 @@ void make_device_tree(const HostTree &t, DeviceTree &out) {
     return (c.b & kLeafBit) ? (kRefLeaf | c.a) : dev[h];
   };
   out.nodes.resize(ninner);
+  out.leaf_start.assign(n, 0);
   out.leaf_end.assign(n, 0);
   for (size_t h = 0; h < nh; ++h) {
     const Node &nd = t.nodes[h];
     if (nd.b & kLeafBit) {
       const uint32_t end = nd.b & ~kLeafBit;
-      for (uint32_t s = nd.a; s < end; ++s) out.leaf_end[s] = end;
+      for (uint32_t s = nd.a; s < end; ++s) {
+        out.leaf_start[s] = nd.a;
+        out.leaf_end[s] = end;
+      }
       continue;
     }
     DNode &d = out.nodes[dev[h]];

@@ -23,11 +23,11 @@ struct HostTree {
 void build_tree(const double *pts, size_t n, int threads, HostTree &out);
 
 // The device tree: the inner nodes of t in pre-order as DNodes, the root's
-// reference, and for every slot the end slot of its leaf.  Throws
+// reference, and for every slot the slot range of its leaf.  Throws
 // std::invalid_argument when the tree exceeds the reference encoding.
 struct DeviceTree {
   std::vector<DNode> nodes;
-  std::vector<uint64_t> leaf_end;
+  std::vector<uint32_t> leaf_start, leaf_end;  // per slot: its leaf's slot range
   uint32_t root = 0;
 };
 void make_device_tree(const HostTree &t, DeviceTree &out);

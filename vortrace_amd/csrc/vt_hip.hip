@@ -497,9 +497,8 @@ struct vt_cloud {
 
   void free_device() {
     for (void *p : {(void *)d_nodes, (void *)d_gen, (void *)d_fields, (void *)d_slot_id,
-                    (void *)d_err, (void *)d_stats, (void *)scratch.knn_tag,
-                    (void *)scratch.knn_val, (void *)scratch.knn_pb, (void *)scratch.walk_tag,
-                    (void *)scratch.walk_s,
+                    (void *)d_err, (void *)d_stats, (void *)scratch.knn_spill,
+                    (void *)scratch.walk_tag, (void *)scratch.walk_s,
                     (void *)scratch.big_id, (void *)scratch.big_d, (void *)d_defer})
       if (p) (void)hipFree(p);
     d_defer = nullptr;
@@ -573,11 +572,18 @@ int leaf_batch() {
   return v;
 }
 
+// Ray partitions of the ray engine (1 or kParts); VT_RAY_PARTS overrides.
+int ray_parts() {
+  static const int v = env_batch("VT_RAY_PARTS", kParts) >= kParts ? kParts : 1;
+  return v;
+}
+
 void ensure_scratch(vt_cloud *c, size_t lanes) {
   c->scratch.walk_batch = walk_batch();
   c->scratch.leaf_batch = leaf_batch();
+  c->scratch.parts = ray_parts();
   if (lanes <= c->scratch_lanes) return;
-  for (void *p : {(void *)c->scratch.knn_tag, (void *)c->scratch.knn_val, (void *)c->scratch.knn_pb,
+  for (void *p : {(void *)c->scratch.knn_spill,
                   (void *)c->scratch.walk_tag, (void *)c->scratch.walk_s,
                   (void *)c->scratch.big_id, (void *)c->scratch.big_d})
     if (p) (void)hipFree(p);
@@ -585,9 +591,7 @@ void ensure_scratch(vt_cloud *c, size_t lanes) {
   s.kspill = c->kspill;
   s.wcap = kWalkCap;
   s.lanes = (int)lanes;
-  s.knn_tag = dev_alloc<uint32_t>((size_t)s.kspill * lanes);
-  s.knn_val = dev_alloc<double>((size_t)s.kspill * lanes);
-  s.knn_pb = dev_alloc<float>((size_t)s.kspill * lanes);
+  s.knn_spill = dev_alloc<uint4>((size_t)s.kspill * lanes);
   s.walk_tag = dev_alloc<uint32_t>((size_t)s.wcap * lanes);
   s.walk_s = dev_alloc<double>((size_t)s.wcap * lanes);
   const size_t waves = (lanes + 63) / 64;
@@ -595,6 +599,7 @@ void ensure_scratch(vt_cloud *c, size_t lanes) {
   s.big_d = dev_alloc<double>(waves * 4 * kMaxQueryK);
   s.walk_batch = walk_batch();
   s.leaf_batch = leaf_batch();
+  s.parts = ray_parts();
   c->scratch = s;
   c->scratch_lanes = lanes;
 }
@@ -931,7 +936,8 @@ int vt_cloud_upload(vt_cloud *c) {
     std::vector<double> fslot(c->n * c->nf);
     for (size_t s = 0; s < c->n; ++s) {
       const size_t i = t.vind[s];
-      gen[s] = Gen{c->pts[3 * i], c->pts[3 * i + 1], c->pts[3 * i + 2], dt.leaf_end[s]};
+      gen[s] = Gen{c->pts[3 * i], c->pts[3 * i + 1], c->pts[3 * i + 2], dt.leaf_start[s],
+                   dt.leaf_end[s]};
       std::memcpy(&fslot[s * c->nf], &c->fields[i * c->nf], c->nf * sizeof(double));
     }
     c->free_device();
@@ -947,14 +953,14 @@ int vt_cloud_upload(vt_cloud *c) {
     if (!dt.nodes.empty())
       HIP_OK(hipMemcpy(c->d_nodes, dt.nodes.data(), dt.nodes.size() * sizeof(DNode),
                        hipMemcpyHostToDevice));
-    gen.resize(c->n + kLeafMaxSize, Gen{0.0, 0.0, 0.0, 0});
+    gen.resize(c->n + kLeafMaxSize, Gen{0.0, 0.0, 0.0, 0xFFFFFFFFu, 0xFFFFFFFFu});
     HIP_OK(hipMemcpy(c->d_gen, gen.data(), gen.size() * sizeof(Gen), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_fields, fslot.data(), std::max<size_t>(1, fslot.size()) * sizeof(double),
                      hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(c->d_slot_id, t.vind.data(), c->n * sizeof(uint32_t),
                      hipMemcpyHostToDevice));
     // kNN stack: at most one entry per level of the current path.
-    c->kspill = std::max<int>(1, (int)t.max_depth + 2 - kLdsDepth);
+    c->kspill = (int)t.max_depth + 2;
     DevCloud &d = c->dc;
     d.nodes = c->d_nodes;
     d.gen = c->d_gen;

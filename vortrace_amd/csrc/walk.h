@@ -54,9 +54,7 @@ struct LaneInfo {
 };
 
 __device__ __forceinline__ KStack make_stack(const Scratch &S, const LaneInfo &L) {
-  return KStack{L.tid,   S.knn_tag + L.gtid, S.knn_val + L.gtid, S.knn_pb + L.gtid,
-                S.lanes, S.kspill,           0,                  0,
-                false,   0};
+  return KStack{L.tid, S.knn_spill + L.gtid, S.lanes, S.kspill, 0, 0, false, 0, 0};
 }
 
 // findSplitPointDistance(pos1, pos2), src/ray.cpp:38-60.
@@ -114,8 +112,7 @@ template <bool kPer>
 __device__ __forceinline__ int classify_expand(const DevCloud &C, const Scratch &S, int tid,
                                             int gtid, double x, double y, double z,
                                             uint32_t ct, uint32_t nt) {
-  KStack st{tid, S.knn_tag + gtid, S.knn_val + gtid, S.knn_pb + gtid, S.lanes, S.kspill,
-            0,   0,                false, 0};
+  KStack st{tid, S.knn_spill + gtid, S.lanes, S.kspill, 0, 0, false, 0, 0};
   const int wave = gtid >> 6;
   const int kb = kExpandK < (int)C.n ? kExpandK : (int)C.n;
   uint32_t *bi = S.big_id + (size_t)wave * 4 * kMaxQueryK;
