@@ -150,8 +150,8 @@ def main():
     pos, fields, start, end = make_workload(args.workload)
     if w["mode"] == "volume":
         # kappa: median ray opacity 0.5 on a 64^2 probe of the same view
-        pc_probe = vt.ProjectionCloud(pos, fields, boundbox=[0, 100] * 3 and
-                                      [0, 100, 0, 100, 0, 100], periodic=True)
+        pc_probe = vt.ProjectionCloud(pos, fields, boundbox=[0, 100, 0, 100, 0, 100],
+                                      periodic=True)
         ps, pe = vt.grid.generate_projection_grid([0.0, 100.0], 64, [0.0, 100.0], None)
         _, ids, se, sx, off = pc_probe.segments_tensor(
             torch.from_numpy(ps.reshape(-1, 3)), torch.from_numpy(pe.reshape(-1, 3)))

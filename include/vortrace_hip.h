@@ -118,6 +118,27 @@ int vt_project_segments(const vt_cloud *cloud, const double *d_start, const doub
                         int64_t *d_cell_ids, double *d_s_enter, double *d_s_exit,
                         void *stream);
 
+/* Replaces the cloud's fields, given on the device in filtered point order
+ * (row i belongs to point vt_cloud_orig_ids()[i]; n x nfields, the nfields of
+ * vt_cloud_create), without rebuilding the tree, and re-applies the RGBA
+ * validity rule of PointCloud::loadPoints (src/pointcloud.cpp:168-179).
+ * No reference counterpart: the reference sets fields only in loadPoints
+ * (src/pointcloud.cpp:12-191); this is the field update of a fitting loop. */
+int vt_cloud_set_fields(vt_cloud *cloud, const double *d_fields, void *stream);
+
+/* Adjoint of vt_project for mode VT_SUM or VT_VOLUME: given d_grad_out
+ * (nrays x n_out, the gradient of a scalar loss with respect to the image),
+ * writes d_grad_fields (n x nfields, filtered point order), the gradient
+ * with respect to the cloud's current fields.  Sum: segment-length-weighted
+ * scatter (the derivative of reduce_sum, src/reduction.cpp:8-21).  Volume:
+ * reverse compositing per ray (the derivative of reduce_volume_render,
+ * src/reduction.cpp:55-83).  The segments are those of vt_project_segments;
+ * per-cell sums run in ray-major segment order (a deterministic segmented
+ * reduction, no floating-point atomics).  No reference counterpart: the
+ * reference has no derivatives. */
+int vt_adjoint(const vt_cloud *cloud, const double *d_start, const double *d_end, size_t nrays,
+               int mode, const double *d_grad_out, double *d_grad_fields, void *stream);
+
 /* Per-call statistics of the last vt_project* call on this thread: total
  * segments (cell crossings), kNN queries run, the deepest walk and kNN
  * stacks, K_EXPAND searches, rays re-run by the K_EXPAND-capable kernel, and

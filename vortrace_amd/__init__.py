@@ -13,6 +13,7 @@ host side.  Importing it without the built library raises ImportError.
 import logging
 
 from . import _lib
+from . import autograd
 from . import grid
 from .core import PointCloud, Projection, ReductionMode, get_verbose, set_verbose
 from .vortrace import ProjectionCloud
@@ -23,4 +24,4 @@ logging.getLogger("vortrace").addHandler(logging.NullHandler())
 
 __version__ = "0.1.0"
 __all__ = ["ProjectionCloud", "PointCloud", "Projection", "ReductionMode",
-           "grid", "set_verbose", "get_verbose", "__version__"]
+           "autograd", "grid", "set_verbose", "get_verbose", "__version__"]

@@ -104,6 +104,7 @@ class PointCloud:
              if vol is not None else np.zeros(0))
         h = _P()
         L = _lib.lib()
+        self._fields_dev = None
         _lib.check(L.vt_cloud_create(_ptr(pos), pos.shape[0], _ptr(fields), nf,
                                      _ptr(sb), _ptr(v) if v.size else None,
                                      v.size, int(bool(periodic)), C.byref(h)))
@@ -156,6 +157,9 @@ class PointCloud:
         return [float(x) for x in self._pos[self.get_orig_ids()[idx]]]
 
     def get_field(self, idx: int, f: int) -> float:
+        fd = getattr(self, "_fields_dev", None)
+        if fd is not None:
+            return float(fd[idx, f].item())
         return float(self._fields[self.get_orig_ids()[idx], f])
 
     def tree_info(self) -> dict:
@@ -183,6 +187,21 @@ class PointCloud:
                  ("inner", int(feat[i]), float(lo[i]), float(hi[i]))
                  for i in range(nn)]
         return vind, bb.reshape(3, 2), nodes
+
+    # -- fields on the device ----------------------------------------------------
+    def set_fields_tensor(self, fields_filtered: torch.Tensor) -> None:
+        """Replaces the fields with a device tensor in filtered point order
+        (n, nfields), keeping the tree (vt_cloud_set_fields)."""
+        if not self.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        nf = self.get_nfields()
+        f = device_f64(fields_filtered, self._device).reshape(-1)
+        if f.numel() != self.get_npart() * nf:
+            raise ValueError(f"fields must have {self.get_npart()} x {nf} values, "
+                             f"got {f.numel()}")
+        _lib.check(_lib.lib().vt_cloud_set_fields(self._handle(), _P(f.data_ptr()),
+                                                  _stream()))
+        self._fields_dev = f.reshape(self.get_npart(), nf)
 
     # -- queries -----------------------------------------------------------------
     def query_tensor(self, q: torch.Tensor, k: int):
@@ -286,6 +305,25 @@ class Projection:
             int(reduction), _P(offsets.data_ptr()), _P(vals.data_ptr()),
             _P(ids.data_ptr()), _P(se.data_ptr()), _P(sx.data_ptr()), _stream()))
         return vals, ids[:total], se[:total], sx[:total], offsets
+
+    def adjoint_tensor(self, cloud: PointCloud, grad_out: torch.Tensor,
+                       reduction=Sum) -> torch.Tensor:
+        """Gradient with respect to the cloud's fields (n, nfields), filtered
+        point order, of a loss whose gradient with respect to this projection's
+        image is grad_out (ngrid, n_out) (vt_adjoint)."""
+        if not cloud.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        s, e = self._rays(cloud)
+        g = device_f64(grad_out, cloud._device).reshape(-1)
+        nout = self._nout(cloud, reduction)
+        if g.numel() != self.ngrid * nout:
+            raise ValueError(f"grad_out must have {self.ngrid} x {nout} values, got {g.numel()}")
+        out = torch.empty((cloud.get_npart(), cloud.get_nfields()), dtype=torch.float64,
+                          device=cloud._device)
+        _lib.check(_lib.lib().vt_adjoint(cloud._handle(), _P(s.data_ptr()), _P(e.data_ptr()),
+                                         self.ngrid, int(reduction), _P(g.data_ptr()),
+                                         _P(out.data_ptr()), _stream()))
+        return out
 
     def makeDetailedProjection(self, cloud: PointCloud, reduction=Sum):
         v, ids, se, sx, off = self.segments_tensor(cloud, reduction)

@@ -9,6 +9,7 @@
 //   k_count      segment counts of makeDetailedProjection
 //   k_fill       segments + values of makeDetailedProjection
 //                                                 (src/projection.cpp:75-144)
+//   k_segs       segment lists for the adjoint (adjoint.h)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,9 +27,13 @@
 #include <thread>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
 #include "../../include/vortrace_hip.h"
-#include "tree_build.h"
+#include "adjoint.h"
 #include "engine.h"
+#include "tree_build.h"
 
 #pragma clang fp contract(off)
 
@@ -359,6 +364,16 @@ __global__ __launch_bounds__(kBlock) void k_fill(DevCloud C, Scratch S, const do
     FillIO<VolIO> io{VolIO{C.fields, out, nout}, offsets, slot_id, ids, se, sx};
     run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
   }
+}
+
+template <bool PER, bool EXP>
+__global__ __launch_bounds__(kBlock) void k_segs(DevCloud C, Scratch S, const double *start,
+                                                 const double *end, int64_t nrays,
+                                                 const uint32_t *ray_list, uint32_t *defer,
+                                                 const int64_t *offsets, uint32_t *slot,
+                                                 double *ds, ErrRec *err, Stats *stats) {
+  SegIO io{offsets, slot, ds};
+  run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
 }
 
 template <int K, bool PER>
@@ -786,6 +801,30 @@ int check_mode(const vt_cloud *c, int mode) {
 
 }  // namespace
 
+namespace {
+
+// Device buffers of one adjoint call.
+struct DevBufs {
+  std::vector<void *> ptrs;
+  template <class T>
+  T *get(size_t count) {
+    T *p = dev_alloc<T>(std::max<size_t>(1, count));
+    ptrs.push_back(p);
+    return p;
+  }
+  ~DevBufs() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+int bits_for(size_t n) {
+  int b = 1;
+  while (b < 32 && ((size_t)1 << b) < n) ++b;
+  return b;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *vt_last_error(void) { return g_err.c_str(); }
@@ -1181,6 +1220,149 @@ int vt_project_segments(const vt_cloud *cc, const double *d_start, const double 
     if (mode == VT_MAX) return go(k_fill<VT_MAX, false, false>, k_fill<VT_MAX, false, true>);
     if (mode == VT_MIN) return go(k_fill<VT_MIN, false, false>, k_fill<VT_MIN, false, true>);
     return go(k_fill<VT_VOLUME, false, false>, k_fill<VT_VOLUME, false, true>);
+  });
+}
+
+int vt_cloud_set_fields(vt_cloud *c, const double *d_fields, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (c->n == 0) return VT_OK;
+    if (!d_fields) return fail(VT_EINVAL, "null argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int *flag = dev_alloc<int>(1);
+    HIP_OK(hipMemsetAsync(flag, 0, sizeof(int), st));
+    const int threads = 256;
+    const int blocks = (int)((c->n + threads - 1) / threads);
+    k_set_fields<<<blocks, threads, 0, st>>>(d_fields, c->d_slot_id, (uint32_t)c->n,
+                                             (uint32_t)c->nf, c->d_fields, flag);
+    HIP_OK(hipGetLastError());
+    int invalid = 0;
+    HIP_OK(hipMemcpyAsync(&invalid, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    (void)hipFree(flag);
+    c->valid_rgba = (c->nf == 4 && !invalid) ? 1 : 0;
+    return VT_OK;
+  });
+}
+
+
+int vt_adjoint(const vt_cloud *cc, const double *d_start, const double *d_end, size_t nrays,
+               int mode, const double *d_grad_out, double *d_grad_fields, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (!rc) rc = check_mode(cc, mode);
+    if (rc) return rc;
+    if (mode != VT_SUM && mode != VT_VOLUME)
+      return fail(VT_EINVAL, "the adjoint is defined for reductions 'sum' and 'volume'");
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nf = (int)c->nf;
+    HIP_OK(hipMemsetAsync(d_grad_fields, 0, c->n * nf * sizeof(double), st));
+    if (nrays == 0) return VT_OK;
+    const int64_t n = (int64_t)nrays;
+    DevBufs B;
+    // 1. segments per ray, and offsets
+    int64_t *counts = B.get<int64_t>(nrays);
+    int64_t *offsets = B.get<int64_t>(nrays + 1);
+    auto count = [&](auto kfast, auto kslow) {
+      return two_pass(
+          c, st, nrays,
+          [&](Timer &tm) {
+            launch(c, kfast, n, st, tm, d_start, d_end, n, (const uint32_t *)nullptr,
+                   c->d_defer, counts, c->d_err, c->d_stats);
+          },
+          [&](Timer &tm, int64_t nd) {
+            launch(c, kslow, nd, st, tm, d_start, d_end, nd, (const uint32_t *)c->d_defer,
+                   (uint32_t *)nullptr, counts, c->d_err, c->d_stats);
+          });
+    };
+    rc = c->periodic ? count(k_count<true, false>, k_count<true, true>)
+                     : count(k_count<false, false>, k_count<false, true>);
+    if (rc) return rc;
+    float ms = g_stats.kernel_ms;
+    HIP_OK(hipMemsetAsync(offsets, 0, sizeof(int64_t), st));
+    {
+      size_t tmp = 0;
+      HIP_OK(rocprim::inclusive_scan(nullptr, tmp, counts, offsets + 1, nrays,
+                                     rocprim::plus<int64_t>(), st));
+      void *t = B.get<char>(tmp);
+      HIP_OK(rocprim::inclusive_scan(t, tmp, counts, offsets + 1, nrays,
+                                     rocprim::plus<int64_t>(), st));
+    }
+    int64_t nseg = 0;
+    HIP_OK(hipMemcpyAsync(&nseg, offsets + nrays, sizeof nseg, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (nseg > (int64_t)0xFFFFFFFFll)
+      return fail(VT_EINVAL, "more than 2^32 - 1 segments in one adjoint call");
+    // 2. segment lists (slot, ds), ray-major
+    uint32_t *slot = B.get<uint32_t>(nseg);
+    double *ds = B.get<double>(nseg);
+    auto segs = [&](auto kfast, auto kslow) {
+      return two_pass(
+          c, st, nrays,
+          [&](Timer &tm) {
+            launch(c, kfast, n, st, tm, d_start, d_end, n, (const uint32_t *)nullptr,
+                   c->d_defer, (const int64_t *)offsets, slot, ds, c->d_err, c->d_stats);
+          },
+          [&](Timer &tm, int64_t nd) {
+            launch(c, kslow, nd, st, tm, d_start, d_end, nd, (const uint32_t *)c->d_defer,
+                   (uint32_t *)nullptr, (const int64_t *)offsets, slot, ds, c->d_err,
+                   c->d_stats);
+          });
+    };
+    rc = c->periodic ? segs(k_segs<true, false>, k_segs<true, true>)
+                     : segs(k_segs<false, false>, k_segs<false, true>);
+    if (rc) return rc;
+    ms += g_stats.kernel_ms;
+    const int64_t segments = g_stats.segments;
+    if (nseg == 0) return VT_OK;
+    Timer tm;
+    HIP_OK(hipEventRecord(tm.e0, st));
+    // 3. contributions per segment
+    const int ncf = mode == VT_VOLUME ? 4 : nf;
+    double *contrib = B.get<double>((size_t)nseg * ncf);
+    const int threads = 256;
+    const int rblocks = (int)((nrays + threads - 1) / threads);
+    if (mode == VT_SUM)
+      k_contrib_sum<<<rblocks, threads, 0, st>>>(n, offsets, ds, d_grad_out, nf, contrib);
+    else
+      k_contrib_vol<<<rblocks, threads, 0, st>>>(n, offsets, slot, ds, c->d_fields, d_grad_out,
+                                                 contrib);
+    HIP_OK(hipGetLastError());
+    // 4. stable sort of the segment indices by slot
+    uint32_t *idx = B.get<uint32_t>(nseg);
+    uint32_t *keys = B.get<uint32_t>(nseg);
+    uint32_t *perm = B.get<uint32_t>(nseg);
+    const int sblocks = (int)((nseg + threads - 1) / threads);
+    k_iota<<<sblocks, threads, 0, st>>>(nseg, idx);
+    HIP_OK(hipGetLastError());
+    {
+      const int nb = bits_for(c->n);
+      size_t tmp = 0;
+      HIP_OK(rocprim::radix_sort_pairs(nullptr, tmp, slot, keys, idx, perm, (size_t)nseg, 0, nb,
+                                       st));
+      void *t = B.get<char>(tmp);
+      HIP_OK(rocprim::radix_sort_pairs(t, tmp, slot, keys, idx, perm, (size_t)nseg, 0, nb, st));
+    }
+    // 5. per-cell sums in ray-major order
+    uint32_t *rs = B.get<uint32_t>(c->n);
+    uint32_t *re = B.get<uint32_t>(c->n);
+    HIP_OK(hipMemsetAsync(rs, 0, c->n * sizeof(uint32_t), st));
+    HIP_OK(hipMemsetAsync(re, 0, c->n * sizeof(uint32_t), st));
+    k_run_bounds<<<sblocks, threads, 0, st>>>(nseg, keys, rs, re);
+    HIP_OK(hipGetLastError());
+    const int cblocks = (int)((c->n + threads - 1) / threads);
+    k_reduce_cells<<<cblocks, threads, 0, st>>>((uint32_t)c->n, ncf, rs, re, perm, contrib,
+                                                 c->d_slot_id, d_grad_fields);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(tm.e1, st));
+    HIP_OK(hipStreamSynchronize(st));
+    float ms2 = 0;
+    HIP_OK(hipEventElapsedTime(&ms2, tm.e0, tm.e1));
+    g_stats.kernel_ms = ms + ms2;
+    g_stats.segments = segments;
+    return VT_OK;
   });
 }
 

@@ -66,7 +66,7 @@ class ProjectionCloud:
                   len(self.pos_orig), len(self.orig_ids))
 
     # ------------------------------------------------------------------
-    def _validate_reduction(self, reduction: str) -> ReductionMode:
+    def _validate_reduction(self, reduction: str, check_rgba: bool = True) -> ReductionMode:
         """vortrace.py:106-125 (same messages)."""
         mode = self._REDUCTION_MAP.get(reduction)
         if mode is None:
@@ -76,7 +76,7 @@ class ProjectionCloud:
             if self._nfields != 4:
                 raise ValueError("Volume rendering requires exactly 4 fields "
                                  f"(R, G, B, alpha), got {self._nfields}")
-            if not self._cloud.get_valid_rgba():
+            if check_rgba and not self._cloud.get_valid_rgba():
                 raise ValueError("Volume rendering requires R, G, B values in "
                                  "[0, 1] and alpha >= 0")
         return mode
@@ -197,6 +197,32 @@ class ProjectionCloud:
             self._cloud, mode)
         oid = torch.from_numpy(self.orig_ids).to(ids.device)
         return v, oid[ids], se, sx, off
+
+    # -- fields on the device and gradients ---------------------------------
+    def _orig_ids_tensor(self, device) -> torch.Tensor:
+        t = getattr(self, "_oid_t", None)
+        if t is None or t.device != device:
+            t = torch.from_numpy(self.orig_ids).to(device)
+            self._oid_t = t
+        return t
+
+    def set_fields(self, fields) -> None:
+        """Replaces the fields ((N,) or (N, nfields), input point order, any
+        device) without rebuilding the tree."""
+        self._require_tree()
+        dev = self._cloud._device
+        f = fields.detach() if isinstance(fields, torch.Tensor) else torch.as_tensor(fields)
+        f = f.to(device=dev, dtype=torch.float64).reshape(len(self.pos_orig), -1)
+        if f.shape[1] != self._nfields:
+            raise ValueError(f"fields must have {self._nfields} columns, got {f.shape[1]}")
+        self._cloud.set_fields_tensor(f[self._orig_ids_tensor(dev)].contiguous())
+
+    def projection_grad(self, pos_start, pos_end, fields, *,
+                        reduction: str = "integrate") -> torch.Tensor:
+        """Projection of `fields` that is differentiable with respect to them
+        (torch.autograd; the backward is the adjoint kernel)."""
+        from .autograd import project
+        return project(self, fields, pos_start, pos_end, reduction)
 
     # -- out of scope here --------------------------------------------------
     def slice(self, *args, **kwargs):
