@@ -6,7 +6,9 @@ Default workload (one step = one full image):
 Inputs (tree, fields, rays) are resident on the device before the timed
 region; tree build and host->device copies are reported separately.
 
-Prints ONE JSON line.  Extra keys: rays_per_s, kernel_ms, roofline (HBM bound,
+Prints ONE JSON line.  Extra keys: rays_per_s, kernel_ms, fwd_adjoint (the
+same rays, forward plus the adjoint of a squared loss through torch.autograd),
+roofline (HBM bound,
 algorithmic bytes per launch / average kernel time from HIP events recorded on
 the kernel's stream), cpu_baseline (the reference's OpenMP C++ core, built from
 its sources into oracle/_ref, timed on this host on a sample of the same rays;
@@ -112,6 +114,62 @@ def cpu_baseline(pos, fields, start, end, periodic, mode, frac, gpu_vals):
             "values_within_1e-12": close}
 
 
+def fwd_adjoint(pc, rho, d_start, d_end, args, C, R):
+    """Forward plus adjoint on the same rays: one step evaluates
+    L = sum over pixels of (proj(rho) - target)^2 and dL/drho through
+    torch.autograd (projection kernel forward, adjoint kernel backward), with
+    target = proj(rho * exp(0.1 N(0,1) per cell))."""
+    import torch
+    import vortrace_amd as vt
+    from vortrace_amd import _lib, synthetic
+    dev = d_start.device
+    rng = np.random.default_rng(55)
+    noisy = rho * np.exp(0.1 * synthetic.normal(rng, len(rho)))
+    pc.set_fields(noisy)
+    target = pc.projection_tensor(d_start, d_end)[:, 0].clone()
+    rho_t = torch.from_numpy(np.ascontiguousarray(rho, dtype=np.float64)).to(dev)
+
+    split = {"forward_ms": [], "backward_ms": []}
+
+    def step():
+        f = rho_t.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        p = vt.autograd.project(pc, f, d_start, d_end, "sum")
+        loss = ((p - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        split["forward_ms"].append(1e3 * (t1 - t0))
+        split["backward_ms"].append(1e3 * (time.perf_counter() - t1))
+        return f.grad, _lib.last_call_stats()
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] fwd+adjoint warmup {i + 1}/{args.warmup} done")
+    split["forward_ms"].clear()
+    split["backward_ms"].clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kms = []
+    for i in range(args.steps):
+        g, st = step()
+        kms.append(st.kernel_ms)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    nz = int(torch.count_nonzero(g).item())
+    fwd_ms, bwd_ms = float(np.mean(split["forward_ms"])), float(np.mean(split["backward_ms"]))
+    log(f"[bench] fwd+adjoint {args.steps} steps: {ms:.2f} ms/step (forward {fwd_ms:.2f}, "
+        f"backward {bwd_ms:.2f}), adjoint kernels {float(np.mean(kms)):.2f} ms, {nz} cells "
+        f"with a gradient")
+    pc.set_fields(rho)
+    return {"ms_per_step": ms, "crossings_per_s": C / (ms * 1e-3), "rays_per_s": R / (ms * 1e-3),
+            "adjoint_kernel_ms": float(np.mean(kms)), "forward_ms": fwd_ms,
+            "backward_ms": bwd_ms, "cells_with_gradient": nz,
+            "step": "L = sum (proj(rho) - target)^2 and dL/drho by torch.autograd "
+                    "(forward kernel + adjoint kernels), FP64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +179,8 @@ def main():
     ap.add_argument("--cpu-sample", type=float, default=0.25,
                     help="fraction of the rays timed on the CPU reference")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the forward+adjoint measurement")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run a plain device-to-device copy of this many GiB "
                          "(reference size for rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -200,6 +260,10 @@ def main():
         f"{st.slow_queries} K_EXPAND, walk depth {st.max_walk_depth}, "
         f"kNN stack {st.max_knn_depth}")
 
+    fa = None
+    if w["mode"] == "sum" and not args.no_adjoint:
+        fa = fwd_adjoint(pc, fields, d_start, d_end, args, C, R)
+
     bytes_alg = algorithmic_bytes(R, C, len(pos), n_f, n_out)
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
     result = {
@@ -226,6 +290,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": read_traffic(args.workload),
                      "algorithmic_bytes_per_launch": bytes_alg},
+        "fwd_adjoint": fa,
         "setup_seconds": {"tree_build_host": info["build_seconds"],
                           "cloud_and_upload": t_build, "rays_h2d": t_h2d},
         "cpu_baseline": None,

@@ -104,6 +104,12 @@ int vt_query_knn(const vt_cloud *cloud, const double *d_q, size_t nq, int k, int
  * VT_VOLUME and nfields otherwise (src/reduction.cpp:98-101). */
 int vt_project(const vt_cloud *cloud, const double *d_start, const double *d_end, size_t nrays,
                int mode, double *d_out, void *stream);
+/* vt_project that also writes every ray's segment count to d_counts (nrays,
+ * the counts of vt_project_count) in the same pass; d_counts may be NULL.
+ * No reference counterpart: the forward pass of a differentiable projection
+ * keeps the counts for vt_adjoint. */
+int vt_project_counted(const vt_cloud *cloud, const double *d_start, const double *d_end,
+                       size_t nrays, int mode, double *d_out, int64_t *d_counts, void *stream);
 
 /* Projection::makeDetailedProjection (src/projection.cpp:75-144) in two
  * passes.  vt_project_count writes the number of segments of every ray to
@@ -132,12 +138,15 @@ int vt_cloud_set_fields(vt_cloud *cloud, const double *d_fields, void *stream);
  * with respect to the cloud's current fields.  Sum: segment-length-weighted
  * scatter (the derivative of reduce_sum, src/reduction.cpp:8-21).  Volume:
  * reverse compositing per ray (the derivative of reduce_volume_render,
- * src/reduction.cpp:55-83).  The segments are those of vt_project_segments;
+ * src/reduction.cpp:55-83).  d_counts: the rays' segment counts from
+ * vt_project_counted, or NULL to count them here.  The segments are those of
+ * vt_project_segments;
  * per-cell sums run in ray-major segment order (a deterministic segmented
  * reduction, no floating-point atomics).  No reference counterpart: the
  * reference has no derivatives. */
 int vt_adjoint(const vt_cloud *cloud, const double *d_start, const double *d_end, size_t nrays,
-               int mode, const double *d_grad_out, double *d_grad_fields, void *stream);
+               int mode, const int64_t *d_counts, const double *d_grad_out,
+               double *d_grad_fields, void *stream);
 
 /* Per-call statistics of the last vt_project* call on this thread: total
  * segments (cell crossings), kNN queries run, the deepest walk and kNN

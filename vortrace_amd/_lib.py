@@ -58,7 +58,8 @@ SIGNATURES = {
     "vt_project_count": (I, [P, P, P, S, P, P]),
     "vt_project_segments": (I, [P, P, P, S, I, P, P, P, P, P, P]),
     "vt_cloud_set_fields": (I, [P, P, P]),
-    "vt_adjoint": (I, [P, P, P, S, I, P, P, P]),
+    "vt_adjoint": (I, [P, P, P, S, I, P, P, P, P]),
+    "vt_project_counted": (I, [P, P, P, S, I, P, P, P]),
     "vt_last_call_stats": (I, [C.POINTER(CallStats)]),
 }
 

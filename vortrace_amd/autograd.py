@@ -4,7 +4,8 @@
 (pos_start, pos_end) through ``pc``'s mesh with ``fields`` -- a tensor in the
 input point order of ``pc``, shape (N,) or (N, nfields) -- and is
 differentiable with respect to ``fields``.  The forward pass is the projection
-kernel; the backward pass is the adjoint kernel (``vt_adjoint``):
+kernel, which also records each ray's segment count; the backward pass is the
+adjoint (``vt_adjoint``), which walks the rays once more for their segments:
 segment-length-weighted scatter for 'sum'/'integrate' (the derivative of the
 reference's reduce_sum, src/reduction.cpp:8-21) and reverse compositing per
 ray for 'volume' (reduce_volume_render, src/reduction.cpp:55-83).  Per-cell
@@ -29,19 +30,20 @@ class ProjectFields(torch.autograd.Function):
     def forward(ctx, fields, pc, pos_start, pos_end, mode):
         pc.set_fields(fields)
         proj = Projection(pos_start, pos_end)
-        out = proj.project_tensor(pc._cloud, mode)
-        ctx.save_for_backward(fields)
+        counts = torch.empty(proj.ngrid, dtype=torch.int64, device=pc._cloud._device)
+        out = proj.project_tensor(pc._cloud, mode, counts=counts)
+        ctx.save_for_backward(fields, counts)
         ctx.pc, ctx.proj, ctx.mode = pc, proj, mode
         return out[:, 0] if out.shape[1] == 1 else out
 
     @staticmethod
     def backward(ctx, grad):
-        (fields,) = ctx.saved_tensors
+        fields, counts = ctx.saved_tensors
         pc, mode = ctx.pc, ctx.mode
         if mode == ReductionMode.VolumeRender:
             pc.set_fields(fields)  # the volume adjoint reads the fields
         g = grad.reshape(grad.shape[0], -1).contiguous()
-        gf = ctx.proj.adjoint_tensor(pc._cloud, g, mode)
+        gf = ctx.proj.adjoint_tensor(pc._cloud, g, mode, counts=counts)
         full = torch.zeros((fields.shape[0], gf.shape[1]), dtype=torch.float64,
                            device=gf.device)
         full[pc._orig_ids_tensor(gf.device)] = gf

@@ -258,17 +258,26 @@ class Projection:
     def _nout(cloud, mode):
         return 3 if int(mode) == VolumeRender else cloud.get_nfields()
 
-    def project_tensor(self, cloud: PointCloud, reduction=Sum) -> torch.Tensor:
-        """makeProjection with the result left on the device (ngrid, n_out)."""
+    def project_tensor(self, cloud: PointCloud, reduction=Sum,
+                       counts: torch.Tensor | None = None) -> torch.Tensor:
+        """makeProjection with the result left on the device (ngrid, n_out);
+        also writes each ray's segment count into `counts` (int64, ngrid)
+        when given."""
         if not cloud.get_tree_built():
             raise RuntimeError("There is currently no valid tree for this point cloud")
         s, e = self._rays(cloud)
         out = torch.zeros((self.ngrid, self._nout(cloud, reduction)),
                           dtype=torch.float64, device=cloud._device)
-        _lib.check(_lib.lib().vt_project(cloud._handle(), _P(s.data_ptr()),
-                                         _P(e.data_ptr()), self.ngrid,
-                                         int(reduction), _P(out.data_ptr()),
-                                         _stream()))
+        cp = None
+        if counts is not None:
+            if counts.dtype != torch.int64 or counts.numel() != self.ngrid or \
+                    not counts.is_contiguous():
+                raise ValueError("counts must be a contiguous int64 tensor of ngrid entries")
+            cp = _P(counts.data_ptr())
+        _lib.check(_lib.lib().vt_project_counted(cloud._handle(), _P(s.data_ptr()),
+                                                 _P(e.data_ptr()), self.ngrid,
+                                                 int(reduction), _P(out.data_ptr()), cp,
+                                                 _stream()))
         return out
 
     def makeProjection(self, cloud: PointCloud, reduction=Sum):
@@ -307,10 +316,11 @@ class Projection:
         return vals, ids[:total], se[:total], sx[:total], offsets
 
     def adjoint_tensor(self, cloud: PointCloud, grad_out: torch.Tensor,
-                       reduction=Sum) -> torch.Tensor:
+                       reduction=Sum, counts: torch.Tensor | None = None) -> torch.Tensor:
         """Gradient with respect to the cloud's fields (n, nfields), filtered
         point order, of a loss whose gradient with respect to this projection's
-        image is grad_out (ngrid, n_out) (vt_adjoint)."""
+        image is grad_out (ngrid, n_out) (vt_adjoint).  `counts`: the rays'
+        segment counts from project_tensor, if kept."""
         if not cloud.get_tree_built():
             raise RuntimeError("There is currently no valid tree for this point cloud")
         s, e = self._rays(cloud)
@@ -321,8 +331,9 @@ class Projection:
         out = torch.empty((cloud.get_npart(), cloud.get_nfields()), dtype=torch.float64,
                           device=cloud._device)
         _lib.check(_lib.lib().vt_adjoint(cloud._handle(), _P(s.data_ptr()), _P(e.data_ptr()),
-                                         self.ngrid, int(reduction), _P(g.data_ptr()),
-                                         _P(out.data_ptr()), _stream()))
+                                         self.ngrid, int(reduction),
+                                         _P(counts.data_ptr()) if counts is not None else None,
+                                         _P(g.data_ptr()), _P(out.data_ptr()), _stream()))
         return out
 
     def makeDetailedProjection(self, cloud: PointCloud, reduction=Sum):

@@ -166,18 +166,27 @@ struct FillSink {  // segments at the ray's offset, plus the reduced values
 // ------------------------------------------------------------ ray I/O ----
 // Per-ray sink set-up and output for run_engine (engine.h).
 
+// Every projection IO also writes the ray's segment count when `counts` is
+// given (the forward pass of a differentiable projection keeps them for the
+// adjoint).
+__device__ __forceinline__ void put_count(int64_t *counts, int64_t ray, uint64_t n) {
+  if (counts) gst(counts + ray, (int64_t)n);
+}
+
 template <int NF>
 struct SumIO {
   using Sink = SumSink<NF>;
   const double *fields;
   double *out;
   int nout;
+  int64_t *counts;
   __device__ void begin(int64_t, Sink &sk) const {
     sk.fields = fields;
     sk.init();
   }
   __device__ int end(int64_t ray, Sink &sk) const {
     sk.store(out + ray * nout);
+    put_count(counts, ray, sk.n);
     return kErrNone;
   }
 };
@@ -188,13 +197,17 @@ struct SumDynIO {
   int nf;
   double *out;
   int nout;
+  int64_t *counts;
   __device__ void begin(int64_t ray, Sink &sk) const {
     sk.fields = fields;
     sk.nf = nf;
     sk.out = out + ray * nout;
     sk.init();
   }
-  __device__ int end(int64_t, Sink &) const { return kErrNone; }
+  __device__ int end(int64_t ray, Sink &sk) const {
+    put_count(counts, ray, sk.n);
+    return kErrNone;
+  }
 };
 
 template <bool kMax>
@@ -204,13 +217,17 @@ struct ExtIO {
   int nf;
   double *out;
   int nout;
+  int64_t *counts;
   __device__ void begin(int64_t ray, Sink &sk) const {
     sk.fields = fields;
     sk.nf = nf;
     sk.out = out + ray * nout;
     sk.init();
   }
-  __device__ int end(int64_t, Sink &) const { return kErrNone; }
+  __device__ int end(int64_t ray, Sink &sk) const {
+    put_count(counts, ray, sk.n);
+    return kErrNone;
+  }
 };
 
 struct VolIO {
@@ -218,12 +235,14 @@ struct VolIO {
   const double *fields;
   double *out;
   int nout;
+  int64_t *counts;
   __device__ void begin(int64_t, Sink &sk) const {
     sk.fields = fields;
     sk.init();
   }
   __device__ int end(int64_t ray, Sink &sk) const {
     sk.store(out + ray * nout);
+    put_count(counts, ray, sk.n);
     return kErrNone;
   }
 };
@@ -300,18 +319,18 @@ template <int MODE, int NF, bool PER>
 __global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const double *start,
                                                     const double *end, int64_t nrays,
                                                     uint32_t *defer, double *out, int nout,
-                                                    ErrRec *err, Stats *stats) {
+                                                    int64_t *counts, ErrRec *err, Stats *stats) {
   if constexpr (MODE == VT_SUM && NF > 0) {
-    SumIO<NF> io{C.fields, out, nout};
+    SumIO<NF> io{C.fields, out, nout, counts};
     run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
   } else if constexpr (MODE == VT_SUM) {
-    SumDynIO io{C.fields, (int)C.nf, out, nout};
+    SumDynIO io{C.fields, (int)C.nf, out, nout, counts};
     run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
   } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
-    ExtIO<MODE == VT_MAX> io{C.fields, (int)C.nf, out, nout};
+    ExtIO<MODE == VT_MAX> io{C.fields, (int)C.nf, out, nout, counts};
     run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
   } else {
-    VolIO io{C.fields, out, nout};
+    VolIO io{C.fields, out, nout, counts};
     run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
   }
 }
@@ -322,16 +341,16 @@ template <int MODE, bool PER>
 __global__ __launch_bounds__(kBlock) void k_project_x(DevCloud C, Scratch S,
                                                       const double *start, const double *end,
                                                       int64_t nrays, const uint32_t *ray_list,
-                                                      double *out, int nout, ErrRec *err,
-                                                      Stats *stats) {
+                                                      double *out, int nout, int64_t *counts,
+                                                      ErrRec *err, Stats *stats) {
   if constexpr (MODE == VT_SUM) {
-    SumDynIO io{C.fields, (int)C.nf, out, nout};
+    SumDynIO io{C.fields, (int)C.nf, out, nout, counts};
     run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
   } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
-    ExtIO<MODE == VT_MAX> io{C.fields, (int)C.nf, out, nout};
+    ExtIO<MODE == VT_MAX> io{C.fields, (int)C.nf, out, nout, counts};
     run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
   } else {
-    VolIO io{C.fields, out, nout};
+    VolIO io{C.fields, out, nout, counts};
     run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
   }
 }
@@ -354,14 +373,15 @@ __global__ __launch_bounds__(kBlock) void k_fill(DevCloud C, Scratch S, const do
                                                  double *se, double *sx, ErrRec *err,
                                                  Stats *stats) {
   if constexpr (MODE == VT_SUM) {
-    FillIO<SumDynIO> io{SumDynIO{C.fields, (int)C.nf, out, nout}, offsets, slot_id, ids, se, sx};
+    FillIO<SumDynIO> io{SumDynIO{C.fields, (int)C.nf, out, nout, nullptr}, offsets, slot_id, ids,
+                        se, sx};
     run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
   } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
-    FillIO<ExtIO<MODE == VT_MAX>> io{ExtIO<MODE == VT_MAX>{C.fields, (int)C.nf, out, nout},
+    FillIO<ExtIO<MODE == VT_MAX>> io{ExtIO<MODE == VT_MAX>{C.fields, (int)C.nf, out, nout, nullptr},
                                      offsets, slot_id, ids, se, sx};
     run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
   } else {
-    FillIO<VolIO> io{VolIO{C.fields, out, nout}, offsets, slot_id, ids, se, sx};
+    FillIO<VolIO> io{VolIO{C.fields, out, nout, nullptr}, offsets, slot_id, ids, se, sx};
     run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
   }
 }
@@ -766,24 +786,24 @@ void launch(vt_cloud *c, K kernel, int64_t n, hipStream_t st, Timer &tm, A... ar
 
 template <int MODE, int NF, bool PER>
 int project_p(vt_cloud *c, const double *s, const double *e, int64_t n, double *out, int nout,
-              hipStream_t st) {
+              int64_t *counts, hipStream_t st) {
   return two_pass(
       c, st, (size_t)n,
       [&](Timer &tm) {
-        launch(c, k_project<MODE, NF, PER>, n, st, tm, s, e, n, c->d_defer, out, nout, c->d_err,
-               c->d_stats);
+        launch(c, k_project<MODE, NF, PER>, n, st, tm, s, e, n, c->d_defer, out, nout, counts,
+               c->d_err, c->d_stats);
       },
       [&](Timer &tm, int64_t nd) {
         launch(c, k_project_x<MODE, PER>, nd, st, tm, s, e, nd, (const uint32_t *)c->d_defer,
-               out, nout, c->d_err, c->d_stats);
+               out, nout, counts, c->d_err, c->d_stats);
       });
 }
 
 template <int MODE, int NF>
 int project(vt_cloud *c, const double *s, const double *e, int64_t n, double *out, int nout,
-            hipStream_t st) {
-  return c->periodic ? project_p<MODE, NF, true>(c, s, e, n, out, nout, st)
-                     : project_p<MODE, NF, false>(c, s, e, n, out, nout, st);
+            int64_t *counts, hipStream_t st) {
+  return c->periodic ? project_p<MODE, NF, true>(c, s, e, n, out, nout, counts, st)
+                     : project_p<MODE, NF, false>(c, s, e, n, out, nout, counts, st);
 }
 
 int check_mode(const vt_cloud *c, int mode) {
@@ -1133,6 +1153,11 @@ int vt_query_knn(const vt_cloud *cc, const double *d_q, size_t nq, int k, int64_
 
 int vt_project(const vt_cloud *cc, const double *d_start, const double *d_end, size_t nrays,
                int mode, double *d_out, void *stream) {
+  return vt_project_counted(cc, d_start, d_end, nrays, mode, d_out, nullptr, stream);
+}
+
+int vt_project_counted(const vt_cloud *cc, const double *d_start, const double *d_end,
+                       size_t nrays, int mode, double *d_out, int64_t *d_counts, void *stream) {
   return guarded([&]() -> int {
     int rc = check_ready(cc);
     if (!rc) rc = check_mode(cc, mode);
@@ -1144,16 +1169,16 @@ int vt_project(const vt_cloud *cc, const double *d_start, const double *d_end, s
     const int64_t n = (int64_t)nrays;
     if (mode == VT_SUM) {
       switch (c->nf) {
-        case 1: return project<VT_SUM, 1>(c, d_start, d_end, n, d_out, nout, st);
-        case 2: return project<VT_SUM, 2>(c, d_start, d_end, n, d_out, nout, st);
-        case 3: return project<VT_SUM, 3>(c, d_start, d_end, n, d_out, nout, st);
-        case 4: return project<VT_SUM, 4>(c, d_start, d_end, n, d_out, nout, st);
-        default: return project<VT_SUM, 0>(c, d_start, d_end, n, d_out, nout, st);
+        case 1: return project<VT_SUM, 1>(c, d_start, d_end, n, d_out, nout, d_counts, st);
+        case 2: return project<VT_SUM, 2>(c, d_start, d_end, n, d_out, nout, d_counts, st);
+        case 3: return project<VT_SUM, 3>(c, d_start, d_end, n, d_out, nout, d_counts, st);
+        case 4: return project<VT_SUM, 4>(c, d_start, d_end, n, d_out, nout, d_counts, st);
+        default: return project<VT_SUM, 0>(c, d_start, d_end, n, d_out, nout, d_counts, st);
       }
     }
-    if (mode == VT_MAX) return project<VT_MAX, 0>(c, d_start, d_end, n, d_out, nout, st);
-    if (mode == VT_MIN) return project<VT_MIN, 0>(c, d_start, d_end, n, d_out, nout, st);
-    return project<VT_VOLUME, 4>(c, d_start, d_end, n, d_out, nout, st);
+    if (mode == VT_MAX) return project<VT_MAX, 0>(c, d_start, d_end, n, d_out, nout, d_counts, st);
+    if (mode == VT_MIN) return project<VT_MIN, 0>(c, d_start, d_end, n, d_out, nout, d_counts, st);
+    return project<VT_VOLUME, 4>(c, d_start, d_end, n, d_out, nout, d_counts, st);
   });
 }
 
@@ -1248,7 +1273,8 @@ int vt_cloud_set_fields(vt_cloud *c, const double *d_fields, void *stream) {
 
 
 int vt_adjoint(const vt_cloud *cc, const double *d_start, const double *d_end, size_t nrays,
-               int mode, const double *d_grad_out, double *d_grad_fields, void *stream) {
+               int mode, const int64_t *d_counts, const double *d_grad_out,
+               double *d_grad_fields, void *stream) {
   return guarded([&]() -> int {
     int rc = check_ready(cc);
     if (!rc) rc = check_mode(cc, mode);
@@ -1277,17 +1303,21 @@ int vt_adjoint(const vt_cloud *cc, const double *d_start, const double *d_end, s
                    (uint32_t *)nullptr, counts, c->d_err, c->d_stats);
           });
     };
-    rc = c->periodic ? count(k_count<true, false>, k_count<true, true>)
-                     : count(k_count<false, false>, k_count<false, true>);
-    if (rc) return rc;
-    float ms = g_stats.kernel_ms;
+    float ms = 0.0f;
+    if (!d_counts) {
+      rc = c->periodic ? count(k_count<true, false>, k_count<true, true>)
+                       : count(k_count<false, false>, k_count<false, true>);
+      if (rc) return rc;
+      ms = g_stats.kernel_ms;
+    }
+    const int64_t *cnt = d_counts ? d_counts : counts;
     HIP_OK(hipMemsetAsync(offsets, 0, sizeof(int64_t), st));
     {
       size_t tmp = 0;
-      HIP_OK(rocprim::inclusive_scan(nullptr, tmp, counts, offsets + 1, nrays,
+      HIP_OK(rocprim::inclusive_scan(nullptr, tmp, cnt, offsets + 1, nrays,
                                      rocprim::plus<int64_t>(), st));
       void *t = B.get<char>(tmp);
-      HIP_OK(rocprim::inclusive_scan(t, tmp, counts, offsets + 1, nrays,
+      HIP_OK(rocprim::inclusive_scan(t, tmp, cnt, offsets + 1, nrays,
                                      rocprim::plus<int64_t>(), st));
     }
     int64_t nseg = 0;
