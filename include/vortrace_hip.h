@@ -148,6 +148,18 @@ int vt_adjoint(const vt_cloud *cloud, const double *d_start, const double *d_end
                int mode, const int64_t *d_counts, const double *d_grad_out,
                double *d_grad_fields, void *stream);
 
+/* Launch parameters of the ray kernels (process-wide): lanes of a wave that
+ * run their walk steps together, lanes that run their leaf steps together
+ * (both 1..64), and ray partitions (1, or 8: one per XCD).  Results do not
+ * depend on them, only speed.  No reference counterpart (the reference's
+ * OpenMP loop has schedule(dynamic,256) fixed, src/projection.cpp:43);
+ * vortrace_amd.tuning.autotune picks them. */
+typedef struct {
+  int walk_batch, leaf_batch, ray_parts;
+} vt_tuning;
+int vt_set_tuning(const vt_tuning *t);
+int vt_get_tuning(vt_tuning *t);
+
 /* Per-call statistics of the last vt_project* call on this thread: total
  * segments (cell crossings), kNN queries run, the deepest walk and kNN
  * stacks, K_EXPAND searches, rays re-run by the K_EXPAND-capable kernel, and

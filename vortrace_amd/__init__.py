@@ -15,6 +15,7 @@ import logging
 from . import _lib
 from . import autograd
 from . import grid
+from . import tuning
 from .core import PointCloud, Projection, ReductionMode, get_verbose, set_verbose
 from .vortrace import ProjectionCloud
 
@@ -24,4 +25,4 @@ logging.getLogger("vortrace").addHandler(logging.NullHandler())
 
 __version__ = "0.1.0"
 __all__ = ["ProjectionCloud", "PointCloud", "Projection", "ReductionMode",
-           "autograd", "grid", "set_verbose", "get_verbose", "__version__"]
+           "autograd", "grid", "tuning", "set_verbose", "get_verbose", "__version__"]

@@ -21,6 +21,10 @@ class TreeInfo(C.Structure):
                 ("max_depth", C.c_int64), ("build_seconds", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("walk_batch", C.c_int), ("leaf_batch", C.c_int), ("ray_parts", C.c_int)]
+
+
 class CallStats(C.Structure):
     _fields_ = [("segments", C.c_int64), ("queries", C.c_int64),
                 ("max_walk_depth", C.c_int64), ("max_knn_depth", C.c_int64),
@@ -60,6 +64,8 @@ SIGNATURES = {
     "vt_cloud_set_fields": (I, [P, P, P]),
     "vt_adjoint": (I, [P, P, P, S, I, P, P, P, P]),
     "vt_project_counted": (I, [P, P, P, S, I, P, P, P]),
+    "vt_set_tuning": (I, [C.POINTER(Tuning)]),
+    "vt_get_tuning": (I, [C.POINTER(Tuning)]),
     "vt_last_call_stats": (I, [C.POINTER(CallStats)]),
 }
 

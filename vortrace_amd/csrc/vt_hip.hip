@@ -590,28 +590,23 @@ int grid_for(K kernel, int64_t items, int dev) {
 }
 
 // Per-lane device scratch for `lanes` persistent lanes.
-// Lanes of a wave that run their walk steps together (engine.h); VT_WALK_BATCH
-// overrides the default.
-int env_batch(const char *name, int dflt) {
+int env_int(const char *name, int dflt, int lo, int hi) {
   const char *e = std::getenv(name);
   const int b = e ? std::atoi(e) : 0;
-  return (b >= 1 && b <= 64) ? b : dflt;
-}
-int walk_batch() {
-  static const int v = env_batch("VT_WALK_BATCH", 32);
-  return v;
-}
-// Lanes of a wave that run their leaf steps together; VT_LEAF_BATCH overrides.
-int leaf_batch() {
-  static const int v = env_batch("VT_LEAF_BATCH", 24);
-  return v;
+  return (b >= lo && b <= hi) ? b : dflt;
 }
 
-// Ray partitions of the ray engine (1 or kParts); VT_RAY_PARTS overrides.
-int ray_parts() {
-  static const int v = env_batch("VT_RAY_PARTS", kParts) >= kParts ? kParts : 1;
-  return v;
-}
+// Ray-engine launch parameters (engine.h): lanes of a wave that run their
+// walk steps / leaf steps together, and ray partitions (1 or kParts).
+// Defaults from a sweep on the 2M-point clustered mesh; VT_WALK_BATCH,
+// VT_LEAF_BATCH and VT_RAY_PARTS override them at load, vt_set_tuning at run
+// time.
+std::atomic<int> g_walk_batch{env_int("VT_WALK_BATCH", 32, 1, 64)};
+std::atomic<int> g_leaf_batch{env_int("VT_LEAF_BATCH", 24, 1, 64)};
+std::atomic<int> g_ray_parts{env_int("VT_RAY_PARTS", kParts, 1, kParts) >= kParts ? kParts : 1};
+int walk_batch() { return g_walk_batch.load(); }
+int leaf_batch() { return g_leaf_batch.load(); }
+int ray_parts() { return g_ray_parts.load(); }
 
 void ensure_scratch(vt_cloud *c, size_t lanes) {
   c->scratch.walk_batch = walk_batch();
@@ -1394,6 +1389,27 @@ int vt_adjoint(const vt_cloud *cc, const double *d_start, const double *d_end, s
     g_stats.segments = segments;
     return VT_OK;
   });
+}
+
+int vt_set_tuning(const vt_tuning *t) {
+  if (!t) return fail(VT_EINVAL, "null argument");
+  if (t->walk_batch < 1 || t->walk_batch > 64 || t->leaf_batch < 1 || t->leaf_batch > 64 ||
+      (t->ray_parts != 1 && t->ray_parts != kParts))
+    return fail(VT_EINVAL, fmt("tuning out of range: walk_batch and leaf_batch in [1, 64], "
+                               "ray_parts 1 or %d",
+                               kParts));
+  g_walk_batch.store(t->walk_batch);
+  g_leaf_batch.store(t->leaf_batch);
+  g_ray_parts.store(t->ray_parts);
+  return VT_OK;
+}
+
+int vt_get_tuning(vt_tuning *t) {
+  if (!t) return fail(VT_EINVAL, "null argument");
+  t->walk_batch = walk_batch();
+  t->leaf_batch = leaf_batch();
+  t->ray_parts = ray_parts();
+  return VT_OK;
 }
 
 int vt_last_call_stats(vt_call_stats *out) {

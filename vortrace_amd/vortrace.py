@@ -187,6 +187,35 @@ class ProjectionCloud:
         self._require_tree()
         return Projection(pos_start, pos_end).project_tensor(self._cloud, mode)
 
+    def batch_projection_tensor(self, views, *, reduction: str = "integrate"):
+        """Projects several ray sets -- views of any sizes and directions,
+        given as (pos_start, pos_end) pairs -- in ONE launch of the ray kernel:
+        every ray joins one global work queue, which evens out the very
+        different crossings per ray of the views.  Returns one device tensor
+        (n_i, n_out) per view."""
+        mode = self._validate_reduction(reduction)
+        self._require_tree()
+        dev = self._cloud._device
+        starts, ends, sizes = [], [], []
+        for ps, pe in views:
+            s = torch.as_tensor(ps, dtype=torch.float64).reshape(-1, 3)
+            e = torch.as_tensor(pe, dtype=torch.float64).reshape(-1, 3)
+            if s.shape != e.shape:
+                raise ValueError("pos_start and pos_end must have the same shape")
+            starts.append(s.to(dev))
+            ends.append(e.to(dev))
+            sizes.append(s.shape[0])
+        if not sizes:
+            return []
+        out = Projection(torch.cat(starts), torch.cat(ends)).project_tensor(self._cloud, mode)
+        return list(torch.split(out, sizes))
+
+    def batch_projection(self, views, *, reduction: str = "integrate"):
+        """batch_projection_tensor with numpy results shaped like
+        ``projection``'s: (n_i,) for one output field, else (n_i, n_out)."""
+        return [o.cpu().numpy()[:, 0].copy() if o.shape[1] == 1 else o.cpu().numpy()
+                for o in self.batch_projection_tensor(views, reduction=reduction)]
+
     def segments_tensor(self, pos_start, pos_end, *,
                         reduction: str = "integrate"):
         """Device-resident traced projection: (values, cell_ids (original
