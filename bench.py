@@ -74,12 +74,14 @@ def algorithmic_bytes(R, C, n_points, n_f, n_out):
 
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE runs, scaled by a copy-kernel calibration), if recorded."""
+    WRITE_SIZE runs, scaled by a copy-kernel calibration) and the L2 hit rate
+    (TCC_HIT / (TCC_HIT + TCC_MISS)) of the same kernel, if recorded."""
     p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if not os.path.exists(p):
-        return None
+        return None, None
     with open(p) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch"), d.get("l2_hit_rate")
 
 
 def cpu_baseline(pos, fields, start, end, periodic, mode, frac, gpu_vals):
@@ -265,6 +267,7 @@ def main():
         fa = fwd_adjoint(pc, fields, d_start, d_end, args, C, R)
 
     bytes_alg = algorithmic_bytes(R, C, len(pos), n_f, n_out)
+    traffic, l2_hit = read_traffic(args.workload)
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
     result = {
         "metric": METRIC,
@@ -288,7 +291,10 @@ def main():
         "knn_queries_per_step": int(st.queries),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic(args.workload),
+                     "traffic": traffic,
+                     "hbm_gbs_measured": (traffic / (kernel_ms * 1e-3) / 1e9
+                                          if traffic else None),
+                     "l2_hit_rate": l2_hit,
                      "algorithmic_bytes_per_launch": bytes_alg},
         "fwd_adjoint": fa,
         "setup_seconds": {"tree_build_host": info["build_seconds"],
