@@ -124,6 +124,24 @@ int vt_project_segments(const vt_cloud *cloud, const double *d_start, const doub
                         int64_t *d_cell_ids, double *d_s_enter, double *d_s_exit,
                         void *stream);
 
+/* PointCloud::saveTreeToBytes / loadTreeFromBytes (src/pointcloud.cpp:243-
+ * 281, bindings 125-132): the host tree as nanoflann 1.3.2's index dump
+ * (include/nanoflann.hpp:1023-1071).  save sets *size and writes the bytes
+ * when buf has room (cap >= *size).  load checks the dump (leaf size 10, a
+ * vind permutation over the cloud's points, leaves tiling the points in
+ * pre-order) and replaces the tree; with upload != 0 it also moves it to the
+ * current device, as vt_cloud_upload. */
+int vt_cloud_save_tree(const vt_cloud *cloud, uint8_t *buf, size_t cap, size_t *size);
+int vt_cloud_load_tree(vt_cloud *cloud, const uint8_t *buf, size_t size, int upload);
+
+/* Slice::makeSlice (src/slice.cpp:7-60): the field values of the nearest cell
+ * at every pixel of an nx x ny grid at z = depth over extent
+ * [xmin, xmax, ymin, ymax], pixel (i, j) at (xmin + dx*i, ymin + dy*j) with
+ * dx = (xmax - xmin) / (nx - 1); d_out is (nx * ny) x nfields, pixel
+ * (i, j) at row i * ny + j. */
+int vt_slice(const vt_cloud *cloud, const double extent[4], size_t nx, size_t ny, double depth,
+             double *d_out, void *stream);
+
 /* Replaces the cloud's fields, given on the device in filtered point order
  * (row i belongs to point vt_cloud_orig_ids()[i]; n x nfields, the nfields of
  * vt_cloud_create), without rebuilding the tree, and re-applies the RGBA

@@ -75,6 +75,58 @@ def test_tree_matches_nanoflann(case, galaxy):
     assert n2 == nodes
 
 
+@pytest.mark.parametrize("case", ["uniform", "lattice"])
+def test_tree_bytes(case):
+    """saveTreeToBytes writes nanoflann's index dump of the same tree; a dump
+    (ours, or the reference's own where oracle/_ref is built) loads back into
+    the same tree; malformed dumps raise ValueError."""
+    if case == "uniform":
+        rng = np.random.default_rng(8)
+        pos, f, bb, vol = rng.random((30000, 3)) * 100, np.ones(30000), [0, 100] * 3, None
+        bb = [0, 100, 0, 100, 0, 100]
+    else:
+        pos, f, vol, bb = cubic_lattice(6)
+    pc = _product_tree(pos, f, bb, vol)
+    data = pc.saveTreeToBytes()
+    vind, bbox, nodes = ref.parse_tree_bytes(data)
+    v2, b2, n2 = pc.tree_export()
+    np.testing.assert_array_equal(vind, v2)
+    np.testing.assert_array_equal(bbox, b2)
+    assert nodes == n2
+    dumps = [data]
+    if ref.available():
+        rc = ref.RefCloud(pos, f, bb, vol=vol)
+        rbytes = rc.tree_bytes()
+        rv, rb, rn = ref.parse_tree_bytes(rbytes)
+        np.testing.assert_array_equal(rv, vind)
+        np.testing.assert_array_equal(rb, bbox)
+        assert rn == nodes
+        assert len(rbytes) == len(data)
+        dumps.append(rbytes)
+    for d in dumps:
+        q = _product_tree(pos, f, bb, vol)
+        q.load_host_tree_bytes(d)
+        v3, b3, n3 = q.tree_export()
+        np.testing.assert_array_equal(v3, v2)
+        np.testing.assert_array_equal(b3, b2)
+        assert n3 == n2
+        assert q.saveTreeToBytes() == data
+    q = _product_tree(pos, f, bb, vol)
+    for bad, msg in [(data[:100], "truncated"), (data[:-40], "truncated"),
+                     (b"\x07" + data[1:], "points"),
+                     (data[:8] + b"\x02" + data[9:], "dimension")]:
+        with pytest.raises(ValueError, match=msg):
+            q.load_host_tree_bytes(bad)
+    empty = PointCloud()
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        empty.loadPoints(np.zeros((0, 3)), np.zeros(0), bb)
+    with pytest.raises(RuntimeError, match="no points loaded"):
+        empty.load_host_tree_bytes(data)
+    with pytest.raises(RuntimeError, match="has not been built"):
+        empty.saveTreeToBytes()
+
+
 def test_cloud_getters_and_filtering():
     pos = np.array([[0.5, 0.5, 0.5], [0.9, 0.9, 0.9], [10.0, 10.0, 10.0]])
     pc = PointCloud()

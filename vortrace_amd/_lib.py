@@ -62,6 +62,9 @@ SIGNATURES = {
     "vt_project_count": (I, [P, P, P, S, P, P]),
     "vt_project_segments": (I, [P, P, P, S, I, P, P, P, P, P, P]),
     "vt_cloud_set_fields": (I, [P, P, P]),
+    "vt_cloud_save_tree": (I, [P, P, S, C.POINTER(S)]),
+    "vt_cloud_load_tree": (I, [P, P, S, I]),
+    "vt_slice": (I, [P, P, S, S, C.c_double, P, P]),
     "vt_adjoint": (I, [P, P, P, S, I, P, P, P, P]),
     "vt_project_counted": (I, [P, P, P, S, I, P, P, P]),
     "vt_set_tuning": (I, [C.POINTER(Tuning)]),

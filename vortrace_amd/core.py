@@ -188,6 +188,31 @@ class PointCloud:
                  for i in range(nn)]
         return vind, bb.reshape(3, 2), nodes
 
+    # -- tree bytes (src/bindings.cpp:125-132) ------------------------------------
+    def saveTreeToBytes(self) -> bytes:
+        """The tree as nanoflann 1.3.2's index dump (vt_cloud_save_tree)."""
+        L = _lib.lib()
+        size = C.c_size_t()
+        _lib.check(L.vt_cloud_save_tree(self._handle(), None, 0, C.byref(size)))
+        buf = (C.c_uint8 * max(1, size.value))()
+        _lib.check(L.vt_cloud_save_tree(self._handle(), buf, size.value, C.byref(size)))
+        return bytes(buf)[: size.value]
+
+    def loadTreeFromBytes(self, data: bytes) -> None:
+        """Replaces the tree with a nanoflann index dump and uploads it to the
+        current device (vt_cloud_load_tree)."""
+        data = bytes(data)
+        buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(data or b"\0")
+        torch.cuda.current_device()
+        _lib.check(_lib.lib().vt_cloud_load_tree(self._handle(), buf, len(data), 1))
+        self._device = torch.device("cuda", torch.cuda.current_device())
+
+    def load_host_tree_bytes(self, data: bytes) -> None:
+        """loadTreeFromBytes without the upload (no device needed)."""
+        data = bytes(data)
+        buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(data or b"\0")
+        _lib.check(_lib.lib().vt_cloud_load_tree(self._handle(), buf, len(data), 0))
+
     # -- fields on the device ----------------------------------------------------
     def set_fields_tensor(self, fields_filtered: torch.Tensor) -> None:
         """Replaces the fields with a device tensor in filtered point order
@@ -345,3 +370,36 @@ class Projection:
         if self._segments is None:
             raise RuntimeError("No detailed projection data available")
         return tuple(t.cpu().numpy() for t in self._segments)
+
+
+class Slice:
+    """Slice (src/slice.hpp, binding src/bindings.cpp:259-275): nearest-cell
+    field values on an nx x ny grid at a fixed depth."""
+
+    def __init__(self, npix, extent, depth):
+        self._npix = (int(npix[0]), int(npix[1]))
+        self._extent = np.ascontiguousarray(np.asarray(extent, dtype=np.float64).ravel())
+        if self._extent.size != 4:
+            raise ValueError("extent must have 4 elements")
+        self._depth = float(depth)
+        self._data = None
+
+    def slice_tensor(self, cloud: PointCloud) -> torch.Tensor:
+        """makeSlice with the result on the device ((nx * ny), nfields)."""
+        if not cloud.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        nx, ny = self._npix
+        out = torch.zeros((nx * ny, cloud.get_nfields()), dtype=torch.float64,
+                          device=cloud._device)
+        _lib.check(_lib.lib().vt_slice(cloud._handle(), _ptr(self._extent), nx, ny,
+                                       C.c_double(self._depth), _P(out.data_ptr()),
+                                       _stream()))
+        return out
+
+    def makeSlice(self, cloud: PointCloud) -> None:
+        self._data = self.slice_tensor(cloud)
+
+    def returnSlice(self) -> np.ndarray:
+        if self._data is None:
+            raise RuntimeError("Slice has not yet been made")
+        return self._data.cpu().numpy()

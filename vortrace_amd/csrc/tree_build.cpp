@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 #include <thread>
 #include <utility>
 
@@ -241,6 +242,144 @@ void make_device_tree(const HostTree &t, DeviceTree &out) {
     d.c2 = ref(nd.a) | ((nd.b & 2u) ? kDimBit : 0u);
   }
   out.root = ref(0);
+}
+
+namespace {
+
+template <class T>
+void put(std::vector<uint8_t> &out, T v) {
+  const size_t o = out.size();
+  out.resize(o + sizeof(T));
+  std::memcpy(out.data() + o, &v, sizeof(T));
+}
+
+struct Reader {
+  const uint8_t *b;
+  size_t size, p = 0;
+  template <class T>
+  T get() {
+    if (size - p < sizeof(T)) throw std::invalid_argument("tree bytes: truncated");
+    T v;
+    std::memcpy(&v, b + p, sizeof(T));
+    p += sizeof(T);
+    return v;
+  }
+};
+
+}  // namespace
+
+void save_tree_bytes(const HostTree &t, std::vector<uint8_t> &out) {
+  out.clear();
+  const uint64_t n = t.vind.size();
+  put<uint64_t>(out, n);
+  put<int32_t>(out, 3);
+  for (int d = 0; d < 3; ++d) {
+    put<double>(out, t.bbox[d][0]);
+    put<double>(out, t.bbox[d][1]);
+  }
+  put<uint64_t>(out, (uint64_t)kLeafMaxSize);
+  put<uint64_t>(out, n);
+  for (uint32_t v : t.vind) put<uint64_t>(out, v);
+  for (const Node &nd : t.nodes) {
+    if (nd.b & kLeafBit) {
+      put<uint64_t>(out, nd.a);
+      put<uint64_t>(out, nd.b & ~kLeafBit);
+      put<uint64_t>(out, 0);
+      put<uint64_t>(out, 0);
+      put<uint64_t>(out, 0);
+    } else {
+      put<int32_t>(out, (int32_t)nd.b);
+      put<int32_t>(out, 0);
+      put<double>(out, nd.lo);
+      put<double>(out, nd.hi);
+      put<uint64_t>(out, 1);
+      put<uint64_t>(out, 1);
+    }
+  }
+}
+
+void load_tree_bytes(const uint8_t *b, size_t size, size_t npart, HostTree &out) {
+  Reader r{b, size};
+  const uint64_t m_size = r.get<uint64_t>();
+  const int32_t dim = r.get<int32_t>();
+  double bbox[3][2];
+  for (int d = 0; d < 3; ++d) {
+    bbox[d][0] = r.get<double>();
+    bbox[d][1] = r.get<double>();
+  }
+  const uint64_t leaf = r.get<uint64_t>();
+  if (m_size != npart)
+    throw std::invalid_argument("tree bytes: the tree indexes " + std::to_string(m_size) +
+                                " points, the cloud has " + std::to_string(npart));
+  if (dim != 3) throw std::invalid_argument("tree bytes: dimension is not 3");
+  if (leaf != (uint64_t)kLeafMaxSize) throw std::invalid_argument("tree bytes: leaf size is not 10");
+  const uint64_t nv = r.get<uint64_t>();
+  if (nv != npart) throw std::invalid_argument("tree bytes: vind size differs from the point count");
+  HostTree t;
+  t.vind.resize(npart);
+  std::vector<uint8_t> seen(npart, 0);
+  for (size_t i = 0; i < npart; ++i) {
+    const uint64_t v = r.get<uint64_t>();
+    if (v >= npart || seen[v]) throw std::invalid_argument("tree bytes: vind is not a permutation");
+    seen[v] = 1;
+    t.vind[i] = (uint32_t)v;
+  }
+  std::memcpy(t.bbox, bbox, sizeof bbox);
+  // Nodes in pre-order.  `open` holds the inner nodes whose child2 has not
+  // started; a leaf ends a subtree, so the next node is child2 of the top.
+  std::vector<size_t> open;
+  std::vector<int64_t> depth_of;
+  uint64_t next_slot = 0;
+  int64_t depth = 0;
+  for (;;) {
+    const size_t i = t.nodes.size();
+    if (i >= (size_t)kMaxNodes * 2) throw std::invalid_argument("tree bytes: too many nodes");
+    uint8_t raw[40];
+    for (int k = 0; k < 40; ++k) raw[k] = r.get<uint8_t>();
+    uint64_t c1, c2;
+    std::memcpy(&c1, raw + 24, 8);
+    std::memcpy(&c2, raw + 32, 8);
+    Node nd{};
+    t.max_depth = std::max(t.max_depth, depth);
+    depth_of.push_back(depth);
+    if (c1 == 0 && c2 == 0) {
+      uint64_t left, right;
+      std::memcpy(&left, raw, 8);
+      std::memcpy(&right, raw + 8, 8);
+      if (left != next_slot || right <= left || right > npart ||
+          right - left > (uint64_t)kLeafMaxSize)
+        throw std::invalid_argument("tree bytes: leaves do not tile the points in order");
+      next_slot = right;
+      nd.a = (uint32_t)left;
+      nd.b = (uint32_t)right | kLeafBit;
+      t.nodes.push_back(nd);
+      t.leaves++;
+      if (open.empty()) break;
+      const size_t j = open.back();
+      open.pop_back();
+      t.nodes[j].a = (uint32_t)(i + 1);
+      depth = depth_of[j] + 1;
+    } else {
+      if (c1 == 0 || c2 == 0) throw std::invalid_argument("tree bytes: inner node with one child");
+      int32_t feat;
+      std::memcpy(&feat, raw, 4);
+      if (feat < 0 || feat > 2) throw std::invalid_argument("tree bytes: cut dimension out of range");
+      std::memcpy(&nd.lo, raw + 8, 8);
+      std::memcpy(&nd.hi, raw + 16, 8);
+      nd.b = (uint32_t)feat;
+      t.nodes.push_back(nd);
+      open.push_back(i);
+      ++depth;
+    }
+  }
+  if (next_slot != npart) throw std::invalid_argument("tree bytes: leaves do not cover every point");
+  for (size_t i = 0; i < t.nodes.size(); ++i) {
+    Node &nd = t.nodes[i];
+    if (nd.b & kLeafBit) continue;
+    t.nodes[i + 1].pb = nd.lo;
+    t.nodes[nd.a].pb = nd.hi;
+  }
+  out = std::move(t);
 }
 
 }  // namespace vt

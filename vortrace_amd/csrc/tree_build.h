@@ -32,4 +32,14 @@ struct DeviceTree {
 };
 void make_device_tree(const HostTree &t, DeviceTree &out);
 
+// nanoflann 1.3.2's index dump (saveIndex / loadIndex,
+// include/nanoflann.hpp:1023-1071): m_size (u64), dim (i32), root_bbox (3 x
+// {low, high} f64), leaf size (u64), vind (u64 count + u64 entries), then the
+// nodes in pre-order as 40-byte Node structs whose child pointers only say
+// null (leaf) or not.  save writes 1 for a non-null child and zeros for
+// padding.  load checks that the dump describes a leaf-size-10 tree over
+// exactly npart points (throws std::invalid_argument otherwise).
+void save_tree_bytes(const HostTree &t, std::vector<uint8_t> &out);
+void load_tree_bytes(const uint8_t *b, size_t size, size_t npart, HostTree &out);
+
 }  // namespace vt

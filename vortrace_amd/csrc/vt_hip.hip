@@ -443,6 +443,36 @@ __global__ __launch_bounds__(kBlock) void k_query_mem(DevCloud C, Scratch S, con
   flush_stats(stats, 0, 0, 0, 0, st.maxsp);
 }
 
+// Slice::makeSlice (src/slice.cpp:7-60): the nearest cell of every pixel of
+// an nx x ny grid at z = depth, pixel (i, j) at (x0 + dx * i, y0 + dy * j);
+// out[(i * ny + j) * nf + f] is that cell's field f.
+template <bool PER>
+__global__ __launch_bounds__(kBlock) void k_slice(DevCloud C, Scratch S, double x0, double dx,
+                                                  double y0, double dy, double depth,
+                                                  int64_t nx, int64_t ny, double *out,
+                                                  ErrRec *err, Stats *stats) {
+  VT_LANE_SETUP(PER)
+  const int64_t np = nx * ny;
+  for (;;) {
+    const int64_t p = (int64_t)atomicAdd(&stats->ray_counter, 1ULL);
+    if (p >= np) break;
+    const int64_t i = p / ny, j = p - i * ny;
+    const double qx = x0 + dx * (double)i;
+    const double qy = y0 + dy * (double)j;
+    uint32_t id[1];
+    double d[1];
+    query_reg<1, PER>(C, qx, qy, depth, id, d, st);
+    if (id[0] == kNone) {
+      report_error(err, kErrNonFinite, p, qx, qy, depth, 0, 0);
+      continue;
+    }
+    for (uint32_t f = 0; f < C.nf; ++f)
+      gst(out + p * C.nf + f, gld(C.fields + (size_t)id[0] * C.nf + f));
+  }
+  if (st.overflow) report_error(err, kErrKnnStack, 0, 0, 0, 0, 0, 0);
+  flush_stats(stats, 0, 0, 0, 0, st.maxsp);
+}
+
 }  // namespace vt
 
 // ================================================================ host ====
@@ -602,7 +632,7 @@ int env_int(const char *name, int dflt, int lo, int hi) {
 // VT_LEAF_BATCH and VT_RAY_PARTS override them at load, vt_set_tuning at run
 // time.
 std::atomic<int> g_walk_batch{env_int("VT_WALK_BATCH", 32, 1, 64)};
-std::atomic<int> g_leaf_batch{env_int("VT_LEAF_BATCH", 24, 1, 64)};
+std::atomic<int> g_leaf_batch{env_int("VT_LEAF_BATCH", 32, 1, 64)};
 std::atomic<int> g_ray_parts{env_int("VT_RAY_PARTS", kParts, 1, kParts) >= kParts ? kParts : 1};
 int walk_batch() { return g_walk_batch.load(); }
 int leaf_batch() { return g_leaf_batch.load(); }
@@ -1240,6 +1270,69 @@ int vt_project_segments(const vt_cloud *cc, const double *d_start, const double 
     if (mode == VT_MAX) return go(k_fill<VT_MAX, false, false>, k_fill<VT_MAX, false, true>);
     if (mode == VT_MIN) return go(k_fill<VT_MIN, false, false>, k_fill<VT_MIN, false, true>);
     return go(k_fill<VT_VOLUME, false, false>, k_fill<VT_VOLUME, false, true>);
+  });
+}
+
+int vt_cloud_save_tree(const vt_cloud *c, uint8_t *buf, size_t cap, size_t *size) {
+  return guarded([&]() -> int {
+    if (!c || !size) return fail(VT_EINVAL, "null argument");
+    if (!c->host_tree) return fail(VT_ERUNTIME, "Cannot save tree: tree has not been built");
+    std::vector<uint8_t> bytes;
+    save_tree_bytes(c->tree, bytes);
+    *size = bytes.size();
+    if (buf && cap >= bytes.size()) std::memcpy(buf, bytes.data(), bytes.size());
+    return VT_OK;
+  });
+}
+
+int vt_cloud_load_tree(vt_cloud *c, const uint8_t *buf, size_t size, int upload) {
+  return guarded([&]() -> int {
+    if (!c || (!buf && size)) return fail(VT_EINVAL, "null argument");
+    if (c->n == 0) return fail(VT_ERUNTIME, "Cannot load tree: no points loaded");
+    HostTree t;
+    load_tree_bytes(buf, size, c->n, t);
+    c->free_device();
+    c->tree_built = false;
+    c->host_tree = true;
+    c->build_seconds = 0.0;
+    c->tree = std::move(t);
+    return upload ? vt_cloud_upload(c) : VT_OK;
+  });
+}
+
+int vt_slice(const vt_cloud *cc, const double extent[4], size_t nx, size_t ny, double depth,
+             double *d_out, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (rc) return rc;
+    if (!extent || (!d_out && nx * ny)) return fail(VT_EINVAL, "null argument");
+    const double *sb = cc->subbox;
+    if ((extent[0] < sb[0]) || (extent[1] > sb[1]) || (extent[2] < sb[2]) ||
+        (extent[3] > sb[3]) || (depth < sb[4]) || (depth > sb[5]))
+      return fail(VT_ERUNTIME, "Slice extent out of bounds of current cloud subbox");
+    if (nx == 0 || ny == 0) return VT_OK;
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    // src/slice.cpp:26-31: deltax = (xmax - xmin) / (npix_x - 1)
+    const double dx = (extent[1] - extent[0]) / (double)(nx - 1);
+    const double dy = (extent[3] - extent[2]) / (double)(ny - 1);
+    const int64_t np = (int64_t)(nx * ny);
+    reset_call(c, st);
+    Timer tm;
+    auto go = [&](auto kernel) {
+      const int grid = grid_for(kernel, np, c->device);
+      ensure_scratch(c, (size_t)grid * kBlock);
+      HIP_OK(hipEventRecord(tm.e0, st));
+      kernel<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, extent[0], dx, extent[2], dy, depth,
+                                      (int64_t)nx, (int64_t)ny, d_out, c->d_err, c->d_stats);
+      HIP_OK(hipGetLastError());
+      HIP_OK(hipEventRecord(tm.e1, st));
+    };
+    if (c->periodic)
+      go(k_slice<true>);
+    else
+      go(k_slice<false>);
+    return finish_call(c, st, tm.e0, tm.e1);
   });
 }
 

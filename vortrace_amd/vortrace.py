@@ -17,7 +17,7 @@ import torch
 from numpy.typing import ArrayLike
 
 from . import grid as gr
-from .core import PointCloud, Projection, ReductionMode
+from .core import PointCloud, Projection, ReductionMode, Slice
 
 _log = logging.getLogger("vortrace")
 
@@ -253,15 +253,27 @@ class ProjectionCloud:
         from .autograd import project
         return project(self, fields, pos_start, pos_end, reduction)
 
-    # -- out of scope here --------------------------------------------------
-    def slice(self, *args, **kwargs):
-        raise NotImplementedError(
-            "slice() (reference src/slice.cpp) is not part of this build")
+    # -- slice (vortrace.py:347-385) ------------------------------------------
+    def slice(self, extent: ArrayLike, nres, depth: float) -> np.ndarray:
+        """Nearest-cell field values on a pixel grid at z = depth (vortrace.py:347-385)."""
+        npix = (nres, nres) if isinstance(nres, int) else tuple(nres)
+        extent_arr = np.asarray(extent, dtype=np.float64).ravel()
+        if extent_arr.size != 4:
+            raise ValueError("extent must have 4 elements [xmin, xmax, ymin, ymax]")
+        sl = Slice(list(npix), list(extent_arr), float(depth))
+        sl.makeSlice(self._cloud)
+        dat = sl.returnSlice()
+        if self._nfields == 1:
+            return dat.reshape(npix[0], npix[1])
+        return dat.reshape(npix[0], npix[1], self._nfields)
 
-    def save_tree_bytes(self):
-        raise NotImplementedError(
-            "nanoflann tree serialisation is not part of this build")
+    # -- tree serialisation (vortrace.py:391-401) -----------------------------
+    def save_tree_bytes(self) -> bytes | None:
+        """The kD-tree as nanoflann's index dump, or None if not built."""
+        if self._cloud.get_tree_built():
+            return self._cloud.saveTreeToBytes()
+        return None
 
-    def load_tree_bytes(self, data):
-        raise NotImplementedError(
-            "nanoflann tree serialisation is not part of this build")
+    def load_tree_bytes(self, data: bytes) -> None:
+        """Restores the kD-tree from a nanoflann index dump."""
+        self._cloud.loadTreeFromBytes(data)
