@@ -123,7 +123,7 @@ def fwd_adjoint(pc, rho, d_start, d_end, args, C, R):
     target = proj(rho * exp(0.1 N(0,1) per cell))."""
     import torch
     import vortrace_amd as vt
-    from vortrace_amd import _lib, synthetic
+    from vortrace_amd import synthetic
     dev = d_start.device
     rng = np.random.default_rng(55)
     noisy = rho * np.exp(0.1 * synthetic.normal(rng, len(rho)))
@@ -144,7 +144,7 @@ def fwd_adjoint(pc, rho, d_start, d_end, args, C, R):
         torch.cuda.synchronize()
         split["forward_ms"].append(1e3 * (t1 - t0))
         split["backward_ms"].append(1e3 * (time.perf_counter() - t1))
-        return f.grad, _lib.last_call_stats()
+        return f.grad, pc._adjoint_stats
 
     for i in range(args.warmup):
         step()
@@ -169,7 +169,8 @@ def fwd_adjoint(pc, rho, d_start, d_end, args, C, R):
             "adjoint_kernel_ms": float(np.mean(kms)), "forward_ms": fwd_ms,
             "backward_ms": bwd_ms, "cells_with_gradient": nz,
             "step": "L = sum (proj(rho) - target)^2 and dL/drho by torch.autograd "
-                    "(forward kernel + adjoint kernels), FP64"}
+                    "(forward kernel recording its segments + adjoint kernels over the "
+                    "records), FP64"}
 
 
 def main():

@@ -166,6 +166,25 @@ int vt_adjoint(const vt_cloud *cloud, const double *d_start, const double *d_end
                int mode, const int64_t *d_counts, const double *d_grad_out,
                double *d_grad_fields, void *stream);
 
+/* The forward and backward of a differentiable projection with one walk of
+ * the rays.  vt_project_recorded is vt_project_counted (modes VT_SUM and
+ * VT_VOLUME) that also appends one VT_SEGREC_BYTES record per segment (its
+ * cell, its ray and its place along the ray, ds) to d_records, a device pool
+ * of `capacity` records, and sets *n_records to the records used, or to
+ * (size_t)-1 when the pool was too small (the image and counts are still
+ * complete).  vt_adjoint_recorded is vt_adjoint over those records: it
+ * scatters them to ray-major order by d_counts and runs the same
+ * contributions and per-cell sums, so the two adjoints agree bit for bit.
+ * The cloud's fields must be those of the forward pass for VT_VOLUME.
+ * No reference counterpart: the reference has no derivatives. */
+#define VT_SEGREC_BYTES 32
+int vt_project_recorded(const vt_cloud *cloud, const double *d_start, const double *d_end,
+                        size_t nrays, int mode, double *d_out, int64_t *d_counts,
+                        void *d_records, size_t capacity, size_t *n_records, void *stream);
+int vt_adjoint_recorded(const vt_cloud *cloud, size_t nrays, int mode, const int64_t *d_counts,
+                        const void *d_records, size_t n_records, const double *d_grad_out,
+                        double *d_grad_fields, void *stream);
+
 /* Launch parameters of the ray kernels (process-wide): lanes of a wave that
  * run their walk steps together, lanes that run their leaf steps together
  * (both 1..64), and ray partitions (1, or 8: one per XCD).  Results do not

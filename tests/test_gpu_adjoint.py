@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 import torch
 
+from conftest import LATTICE_RAYS, cubic_lattice
 from oracle import port
 
 pytestmark = pytest.mark.gpu
@@ -154,3 +155,86 @@ def test_adjoint_is_deterministic_and_checks_mode():
     assert np.count_nonzero(a) > 0
     with pytest.raises(ValueError, match="adjoint"):
         p.adjoint_tensor(pc._cloud, g, vt.ReductionMode.Max)
+
+
+def _recorded_case(case):
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(31)
+    if case == "lattice":
+        # The reference's degenerate rays, repeated so that many lanes and
+        # record chunks carry them; K_EXPAND rays run in the second launch.
+        pos, _, vol, bb = cubic_lattice()
+        f = 0.5 + rng.random(len(pos))
+        pc = vt.ProjectionCloud(pos, f, boundbox=bb, vol=vol)
+        s = np.array([r[0] for r in LATTICE_RAYS] * 40, float)
+        e = np.array([r[1] for r in LATTICE_RAYS] * 40, float)
+        mode = vt.ReductionMode.Sum
+    else:
+        pos, rho = synthetic.clustered_mesh(40_000, seed=32, n_clumps=12)
+        bb = [0, 100, 0, 100, 0, 100]
+        if case == "volume":
+            f = np.empty((len(pos), 4))
+            f[:, :3] = 0.1 + 0.8 * rng.random((len(pos), 3))
+            f[:, 3] = 0.02 + 0.05 * rng.random(len(pos))
+            mode = vt.ReductionMode.VolumeRender
+        else:
+            f = np.column_stack([rho, rng.random(len(pos))])
+            mode = vt.ReductionMode.Sum
+        pc = vt.ProjectionCloud(pos, f, boundbox=bb, periodic=True)
+        s, e = _rays(40)
+    return pc, f, torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev), mode
+
+
+@pytest.mark.parametrize("case", ["sum", "volume", "lattice"])
+def test_recorded_adjoint_matches_segment_walk(case):
+    """One-walk forward/backward (vt_project_recorded, vt_adjoint_recorded)
+    against the projection and the segment-walk adjoint: the same bits;
+    an overflowing pool reports itself and leaves image and counts whole."""
+    _say(f"recorded adjoint case={case}")
+    pc, _, ds_, de_, mode = _recorded_case(case)
+    dev = ds_.device
+    p = vt.Projection(ds_, de_)
+    n = p.ngrid
+    c_ref = torch.empty(n, dtype=torch.int64, device=dev)
+    img = p.project_tensor(pc._cloud, mode, counts=c_ref)
+    total = int(c_ref.sum().item())
+    counts = torch.empty(n, dtype=torch.int64, device=dev)
+    pool = torch.empty((total + 2 * 131072 * 32, vt.core.SEGREC_BYTES), dtype=torch.uint8,
+                       device=dev)
+    img_r, used = p.project_recorded_tensor(pc._cloud, mode, counts, pool)
+    deferred = vt._lib.last_call_stats().deferred_rays
+    _say(f"  {n} rays, {total} segments, {used} records, {deferred} deferred rays")
+    if case == "lattice":
+        assert deferred > 0
+    assert total <= used <= pool.shape[0]
+    assert torch.equal(img_r, img) and torch.equal(counts, c_ref)
+    g = torch.from_numpy(np.random.default_rng(33).standard_normal(img.shape)).to(dev)
+    a = p.adjoint_tensor(pc._cloud, g, mode, counts=c_ref)
+    b = p.adjoint_recorded_tensor(pc._cloud, g, mode, counts, pool, used)
+    assert torch.equal(a, b)
+    assert int(torch.count_nonzero(b)) > 0
+    small = torch.empty((total // 2, vt.core.SEGREC_BYTES), dtype=torch.uint8, device=dev)
+    counts.fill_(-1)
+    img_o, used_o = p.project_recorded_tensor(pc._cloud, mode, counts, small)
+    assert used_o == -1
+    assert torch.equal(img_o, img) and torch.equal(counts, c_ref)
+    with pytest.raises(ValueError, match="overflowed"):
+        p.adjoint_recorded_tensor(pc._cloud, g, mode, counts, small, used_o)
+
+
+def test_autograd_records_after_first_pass():
+    """The autograd wrapper records segments from its second forward pass on
+    (pool sized from the first); gradients keep their bits."""
+    _say("autograd recorded passes")
+    pc, f_np, ds_, de_, _ = _recorded_case("sum")
+    f0 = torch.from_numpy(f_np).to(ds_.device)
+    w = torch.from_numpy(np.random.default_rng(35).standard_normal((ds_.shape[0], 2))).to(
+        ds_.device)
+    grads = []
+    for _ in range(3):
+        f = f0.clone().requires_grad_(True)
+        (vt.autograd.project(pc, f, ds_, de_, "sum") * w).sum().backward()
+        grads.append(f.grad)
+        assert pc._segment_estimate > 0
+    assert torch.equal(grads[0], grads[1]) and torch.equal(grads[1], grads[2])
+    assert int(torch.count_nonzero(grads[0])) > 0

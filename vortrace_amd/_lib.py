@@ -67,6 +67,8 @@ SIGNATURES = {
     "vt_slice": (I, [P, P, S, S, C.c_double, P, P]),
     "vt_adjoint": (I, [P, P, P, S, I, P, P, P, P]),
     "vt_project_counted": (I, [P, P, P, S, I, P, P, P]),
+    "vt_project_recorded": (I, [P, P, P, S, I, P, P, P, S, C.POINTER(S), P]),
+    "vt_adjoint_recorded": (I, [P, S, I, P, P, S, P, P, P]),
     "vt_set_tuning": (I, [C.POINTER(Tuning)]),
     "vt_get_tuning": (I, [C.POINTER(Tuning)]),
     "vt_last_call_stats": (I, [C.POINTER(CallStats)]),

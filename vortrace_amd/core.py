@@ -54,6 +54,15 @@ def _stream():
     return _P(torch.cuda.current_stream().cuda_stream)
 
 
+# Bytes of one segment record of vt_project_recorded (VT_SEGREC_BYTES).
+SEGREC_BYTES = 32
+
+
+def _check_counts(counts: torch.Tensor, ngrid: int) -> None:
+    if counts.dtype != torch.int64 or counts.numel() != ngrid or not counts.is_contiguous():
+        raise ValueError("counts must be a contiguous int64 tensor of ngrid entries")
+
+
 def device_f64(a, device) -> torch.Tensor:
     """A contiguous FP64 tensor on `device` (copied only when needed)."""
     if isinstance(a, torch.Tensor):
@@ -295,14 +304,60 @@ class Projection:
                           dtype=torch.float64, device=cloud._device)
         cp = None
         if counts is not None:
-            if counts.dtype != torch.int64 or counts.numel() != self.ngrid or \
-                    not counts.is_contiguous():
-                raise ValueError("counts must be a contiguous int64 tensor of ngrid entries")
+            _check_counts(counts, self.ngrid)
             cp = _P(counts.data_ptr())
         _lib.check(_lib.lib().vt_project_counted(cloud._handle(), _P(s.data_ptr()),
                                                  _P(e.data_ptr()), self.ngrid,
                                                  int(reduction), _P(out.data_ptr()), cp,
                                                  _stream()))
+        return out
+
+    def project_recorded_tensor(self, cloud: PointCloud, reduction, counts: torch.Tensor,
+                                records: torch.Tensor):
+        """project_tensor for 'sum' or 'volume' that also records every
+        segment into `records` (uint8, (capacity, SEGREC_BYTES), on the
+        device) for adjoint_recorded_tensor (vt_project_recorded).  Returns
+        (image, records used), records used -1 when the pool was too small."""
+        if not cloud.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        _check_counts(counts, self.ngrid)
+        if records.dtype != torch.uint8 or records.dim() != 2 or \
+                records.shape[1] != SEGREC_BYTES or not records.is_contiguous():
+            raise ValueError(f"records must be a contiguous uint8 tensor (capacity, "
+                             f"{SEGREC_BYTES})")
+        s, e = self._rays(cloud)
+        out = torch.zeros((self.ngrid, self._nout(cloud, reduction)),
+                          dtype=torch.float64, device=cloud._device)
+        used = C.c_size_t(0)
+        _lib.check(_lib.lib().vt_project_recorded(
+            cloud._handle(), _P(s.data_ptr()), _P(e.data_ptr()), self.ngrid, int(reduction),
+            _P(out.data_ptr()), _P(counts.data_ptr()), _P(records.data_ptr()),
+            records.shape[0], C.byref(used), _stream()))
+        n = used.value
+        return out, (-1 if n == C.c_size_t(-1).value else n)
+
+    def adjoint_recorded_tensor(self, cloud: PointCloud, grad_out: torch.Tensor, reduction,
+                                counts: torch.Tensor, records: torch.Tensor,
+                                n_records: int) -> torch.Tensor:
+        """adjoint_tensor from the records of project_recorded_tensor
+        (vt_adjoint_recorded); the same bits as adjoint_tensor."""
+        if not cloud.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        _check_counts(counts, self.ngrid)
+        if n_records < 0:
+            raise ValueError("the recorded projection overflowed its record pool")
+        if n_records > records.shape[0]:
+            raise ValueError("n_records exceeds the pool")
+        g = device_f64(grad_out, cloud._device).reshape(-1)
+        nout = self._nout(cloud, reduction)
+        if g.numel() != self.ngrid * nout:
+            raise ValueError(f"grad_out must have {self.ngrid} x {nout} values, got {g.numel()}")
+        out = torch.empty((cloud.get_npart(), cloud.get_nfields()), dtype=torch.float64,
+                          device=cloud._device)
+        _lib.check(_lib.lib().vt_adjoint_recorded(
+            cloud._handle(), self.ngrid, int(reduction), _P(counts.data_ptr()),
+            _P(records.data_ptr()), n_records, _P(g.data_ptr()), _P(out.data_ptr()),
+            _stream()))
         return out
 
     def makeProjection(self, cloud: PointCloud, reduction=Sum):

@@ -53,6 +53,84 @@ struct SegIO {
   }
 };
 
+// One segment recorded by a forward pass: the cell's slot, the ray, its
+// position k along the ray, and ds.  ray == kNone marks a record never
+// written (the pool is filled with 0xFF first).
+struct alignas(16) SegRec {
+  uint32_t slot, ray, k, pad;
+  double ds, pad2;
+};
+static_assert(sizeof(SegRec) == 32, "segment record must be 32 bytes");
+
+// Pool the records go to: lanes take chunks of kRecChunk records from `head`
+// and fill them in order; `overflow` is set when the pool runs out.
+struct RecPool {
+  SegRec *rec;
+  unsigned long long cap;
+  unsigned long long *head;
+  int *overflow;
+};
+constexpr int kRecChunk = 32;
+
+// A projection IO that also records every segment.  The sink's chunk state
+// survives from ray to ray (begin() leaves it alone).  A ray left to the
+// K_EXPAND launch records a prefix of its segments twice, with the same
+// values: the backward pass writes both to the same place.
+template <class InnerIO>
+struct RecordIO {
+  struct Sink {
+    typename InnerIO::Sink in;
+    RecPool pool;
+    uint32_t ray;
+    uint64_t n;
+    unsigned long long base = 0;
+    int fill = kRecChunk;
+    __device__ __forceinline__ void seg(uint32_t cell, double a, double b) {
+      in.seg(cell, a, b);
+      if (fill == kRecChunk) {
+        base = atomicAdd(pool.head, (unsigned long long)kRecChunk);
+        fill = 0;
+      }
+      const unsigned long long i = base + (unsigned long long)fill;
+      if (i < pool.cap) {
+        SegRec *r = pool.rec + i;
+        gst(reinterpret_cast<uint4 *>(r), make_uint4(cell, ray, (uint32_t)n, 0u));
+        gst(&r->ds, b - a);
+      } else {
+        atomicOr(pool.overflow, 1);
+      }
+      ++fill;
+      ++n;
+    }
+  };
+  InnerIO inner;
+  RecPool pool;
+  __device__ void begin(int64_t ray, Sink &sk) const {
+    inner.begin(ray, sk.in);
+    sk.pool = pool;
+    sk.ray = (uint32_t)ray;
+    sk.n = 0;
+  }
+  __device__ int end(int64_t ray, Sink &sk) const { return inner.end(ray, sk.in); }
+};
+
+// Recorded segments -> the ray-major (slot, ds) lists at offsets[ray] + k.
+// Unwritten records (ray == kNone) and records past a ray's count are
+// skipped; every (ray, k < counts[ray]) is written by at least one record
+// when the pool did not overflow.
+__global__ void k_scatter_records(uint64_t nrec, const SegRec *rec, int64_t nrays,
+                                  const int64_t *offsets, const int64_t *counts, uint32_t *slot,
+                                  double *ds) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nrec) return;
+  const uint4 h = gld(reinterpret_cast<const uint4 *>(rec + j));
+  if (h.y == kNone || (int64_t)h.y >= nrays) return;
+  if ((int64_t)h.z >= gld(counts + h.y)) return;
+  const int64_t pos = gld(offsets + h.y) + h.z;
+  gst(slot + pos, h.x);
+  gst(ds + pos, gld(&rec[j].ds));
+}
+
 // Fields given in filtered point order -> slot order, and the RGBA validity
 // rule of PointCloud::loadPoints (src/pointcloud.cpp:168-179): a value
 // outside [0, 1] in R, G or B, or a negative alpha, makes the set invalid.

@@ -315,43 +315,59 @@ __device__ __forceinline__ void run_rays(const DevCloud &C, const Scratch &S, co
 // fast variants (EXP = false) append rays that need the K_EXPAND
 // classification to `defer`; the EXP = true variants handle every ray.
 
-template <int MODE, int NF, bool PER>
+// With REC every segment is also recorded to `pool` (differentiable forward).
+template <bool REC, bool PER, bool EXP, class IO>
+__device__ __forceinline__ void run_io(const DevCloud &C, const Scratch &S, const double *start,
+                                       const double *end, int64_t nrays,
+                                       const uint32_t *ray_list, uint32_t *defer, ErrRec *err,
+                                       Stats *stats, const IO &io, const RecPool &pool) {
+  if constexpr (REC) {
+    RecordIO<IO> rio{io, pool};
+    run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, rio);
+  } else {
+    IO cio = io;
+    run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, cio);
+  }
+}
+
+template <int MODE, int NF, bool PER, bool REC = false>
 __global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const double *start,
                                                     const double *end, int64_t nrays,
                                                     uint32_t *defer, double *out, int nout,
-                                                    int64_t *counts, ErrRec *err, Stats *stats) {
+                                                    int64_t *counts, RecPool pool, ErrRec *err,
+                                                    Stats *stats) {
   if constexpr (MODE == VT_SUM && NF > 0) {
-    SumIO<NF> io{C.fields, out, nout, counts};
-    run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
+    run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
+                            SumIO<NF>{C.fields, out, nout, counts}, pool);
   } else if constexpr (MODE == VT_SUM) {
-    SumDynIO io{C.fields, (int)C.nf, out, nout, counts};
-    run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
+    run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
+                            SumDynIO{C.fields, (int)C.nf, out, nout, counts}, pool);
   } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
-    ExtIO<MODE == VT_MAX> io{C.fields, (int)C.nf, out, nout, counts};
-    run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
+    run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
+                            ExtIO<MODE == VT_MAX>{C.fields, (int)C.nf, out, nout, counts}, pool);
   } else {
-    VolIO io{C.fields, out, nout, counts};
-    run_rays<PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats, io);
+    run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
+                            VolIO{C.fields, out, nout, counts}, pool);
   }
 }
 
 // The deferred rays of k_project.  Sums accumulate in place, in the same order
 // as SumSink's registers, so the values are the same bits.
-template <int MODE, bool PER>
+template <int MODE, bool PER, bool REC = false>
 __global__ __launch_bounds__(kBlock) void k_project_x(DevCloud C, Scratch S,
                                                       const double *start, const double *end,
                                                       int64_t nrays, const uint32_t *ray_list,
                                                       double *out, int nout, int64_t *counts,
-                                                      ErrRec *err, Stats *stats) {
+                                                      RecPool pool, ErrRec *err, Stats *stats) {
   if constexpr (MODE == VT_SUM) {
-    SumDynIO io{C.fields, (int)C.nf, out, nout, counts};
-    run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
+    run_io<REC, PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats,
+                           SumDynIO{C.fields, (int)C.nf, out, nout, counts}, pool);
   } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
-    ExtIO<MODE == VT_MAX> io{C.fields, (int)C.nf, out, nout, counts};
-    run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
+    run_io<REC, PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats,
+                           ExtIO<MODE == VT_MAX>{C.fields, (int)C.nf, out, nout, counts}, pool);
   } else {
-    VolIO io{C.fields, out, nout, counts};
-    run_rays<PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats, io);
+    run_io<REC, PER, true>(C, S, start, end, nrays, ray_list, nullptr, err, stats,
+                           VolIO{C.fields, out, nout, counts}, pool);
   }
 }
 
@@ -809,26 +825,26 @@ void launch(vt_cloud *c, K kernel, int64_t n, hipStream_t st, Timer &tm, A... ar
   HIP_OK(hipEventRecord(tm.e1, st));
 }
 
-template <int MODE, int NF, bool PER>
+template <int MODE, int NF, bool PER, bool REC>
 int project_p(vt_cloud *c, const double *s, const double *e, int64_t n, double *out, int nout,
-              int64_t *counts, hipStream_t st) {
+              int64_t *counts, const RecPool &pool, hipStream_t st) {
   return two_pass(
       c, st, (size_t)n,
       [&](Timer &tm) {
-        launch(c, k_project<MODE, NF, PER>, n, st, tm, s, e, n, c->d_defer, out, nout, counts,
-               c->d_err, c->d_stats);
+        launch(c, k_project<MODE, NF, PER, REC>, n, st, tm, s, e, n, c->d_defer, out, nout,
+               counts, pool, c->d_err, c->d_stats);
       },
       [&](Timer &tm, int64_t nd) {
-        launch(c, k_project_x<MODE, PER>, nd, st, tm, s, e, nd, (const uint32_t *)c->d_defer,
-               out, nout, counts, c->d_err, c->d_stats);
+        launch(c, k_project_x<MODE, PER, REC>, nd, st, tm, s, e, nd,
+               (const uint32_t *)c->d_defer, out, nout, counts, pool, c->d_err, c->d_stats);
       });
 }
 
-template <int MODE, int NF>
+template <int MODE, int NF, bool REC = false>
 int project(vt_cloud *c, const double *s, const double *e, int64_t n, double *out, int nout,
-            int64_t *counts, hipStream_t st) {
-  return c->periodic ? project_p<MODE, NF, true>(c, s, e, n, out, nout, counts, st)
-                     : project_p<MODE, NF, false>(c, s, e, n, out, nout, counts, st);
+            int64_t *counts, hipStream_t st, const RecPool &pool = RecPool{}) {
+  return c->periodic ? project_p<MODE, NF, true, REC>(c, s, e, n, out, nout, counts, pool, st)
+                     : project_p<MODE, NF, false, REC>(c, s, e, n, out, nout, counts, pool, st);
 }
 
 int check_mode(const vt_cloud *c, int mode) {
@@ -866,6 +882,72 @@ int bits_for(size_t n) {
   int b = 1;
   while (b < 32 && ((size_t)1 << b) < n) ++b;
   return b;
+}
+
+// Steps after the segment lists of an adjoint call: contributions per
+// segment, the stable sort of segment indices by slot, and the per-cell sums
+// in ray-major order.  Returns the GPU time of these steps in ms.
+float adjoint_tail(vt_cloud *c, int mode, int64_t n, const int64_t *offsets,
+                   const uint32_t *slot, const double *ds, int64_t nseg,
+                   const double *d_grad_out, double *d_grad_fields, DevBufs &B, hipStream_t st) {
+  const int nf = (int)c->nf;
+  Timer tm;
+  HIP_OK(hipEventRecord(tm.e0, st));
+  const int ncf = mode == VT_VOLUME ? 4 : nf;
+  double *contrib = B.get<double>((size_t)nseg * ncf);
+  const int threads = 256;
+  const int rblocks = (int)((n + threads - 1) / threads);
+  if (mode == VT_SUM)
+    k_contrib_sum<<<rblocks, threads, 0, st>>>(n, offsets, ds, d_grad_out, nf, contrib);
+  else
+    k_contrib_vol<<<rblocks, threads, 0, st>>>(n, offsets, slot, ds, c->d_fields, d_grad_out,
+                                               contrib);
+  HIP_OK(hipGetLastError());
+  uint32_t *idx = B.get<uint32_t>(nseg);
+  uint32_t *keys = B.get<uint32_t>(nseg);
+  uint32_t *perm = B.get<uint32_t>(nseg);
+  const int sblocks = (int)((nseg + threads - 1) / threads);
+  k_iota<<<sblocks, threads, 0, st>>>(nseg, idx);
+  HIP_OK(hipGetLastError());
+  {
+    const int nb = bits_for(c->n);
+    size_t tmp = 0;
+    HIP_OK(rocprim::radix_sort_pairs(nullptr, tmp, slot, keys, idx, perm, (size_t)nseg, 0, nb,
+                                     st));
+    void *t = B.get<char>(tmp);
+    HIP_OK(rocprim::radix_sort_pairs(t, tmp, slot, keys, idx, perm, (size_t)nseg, 0, nb, st));
+  }
+  uint32_t *rs = B.get<uint32_t>(c->n);
+  uint32_t *re = B.get<uint32_t>(c->n);
+  HIP_OK(hipMemsetAsync(rs, 0, c->n * sizeof(uint32_t), st));
+  HIP_OK(hipMemsetAsync(re, 0, c->n * sizeof(uint32_t), st));
+  k_run_bounds<<<sblocks, threads, 0, st>>>(nseg, keys, rs, re);
+  HIP_OK(hipGetLastError());
+  const int cblocks = (int)((c->n + threads - 1) / threads);
+  k_reduce_cells<<<cblocks, threads, 0, st>>>((uint32_t)c->n, ncf, rs, re, perm, contrib,
+                                               c->d_slot_id, d_grad_fields);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipEventRecord(tm.e1, st));
+  HIP_OK(hipStreamSynchronize(st));
+  float ms = 0;
+  HIP_OK(hipEventElapsedTime(&ms, tm.e0, tm.e1));
+  return ms;
+}
+
+// Offsets (exclusive scan with the total at [n]) of per-ray segment counts;
+// returns the total.
+int64_t scan_counts(const int64_t *cnt, size_t nrays, int64_t *offsets, DevBufs &B,
+                    hipStream_t st) {
+  HIP_OK(hipMemsetAsync(offsets, 0, sizeof(int64_t), st));
+  size_t tmp = 0;
+  HIP_OK(rocprim::inclusive_scan(nullptr, tmp, cnt, offsets + 1, nrays,
+                                 rocprim::plus<int64_t>(), st));
+  void *t = B.get<char>(tmp);
+  HIP_OK(rocprim::inclusive_scan(t, tmp, cnt, offsets + 1, nrays, rocprim::plus<int64_t>(), st));
+  int64_t nseg = 0;
+  HIP_OK(hipMemcpyAsync(&nseg, offsets + nrays, sizeof nseg, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  return nseg;
 }
 
 }  // namespace
@@ -1207,6 +1289,55 @@ int vt_project_counted(const vt_cloud *cc, const double *d_start, const double *
   });
 }
 
+static_assert(sizeof(SegRec) == VT_SEGREC_BYTES, "record size of the C interface");
+
+int vt_project_recorded(const vt_cloud *cc, const double *d_start, const double *d_end,
+                        size_t nrays, int mode, double *d_out, int64_t *d_counts,
+                        void *d_records, size_t capacity, size_t *n_records, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (!rc) rc = check_mode(cc, mode);
+    if (rc) return rc;
+    if (mode != VT_SUM && mode != VT_VOLUME)
+      return fail(VT_EINVAL, "recorded projections are for reductions 'sum' and 'volume'");
+    if (!d_counts || !n_records || (!d_records && capacity))
+      return fail(VT_EINVAL, "null argument");
+    if (reinterpret_cast<uintptr_t>(d_records) % alignof(SegRec))
+      return fail(VT_EINVAL, "the record pool must be 16-byte aligned");
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    *n_records = 0;
+    if (nrays == 0) return VT_OK;
+    const int nout = mode == VT_VOLUME ? 3 : (int)c->nf;
+    const int64_t n = (int64_t)nrays;
+    DevBufs B;
+    unsigned long long *head = B.get<unsigned long long>(1);
+    int *overflow = B.get<int>(1);
+    HIP_OK(hipMemsetAsync(head, 0, sizeof *head, st));
+    HIP_OK(hipMemsetAsync(overflow, 0, sizeof *overflow, st));
+    if (capacity) HIP_OK(hipMemsetAsync(d_records, 0xFF, capacity * sizeof(SegRec), st));
+    const RecPool pool{static_cast<SegRec *>(d_records), (unsigned long long)capacity, head,
+                       overflow};
+    if (mode == VT_SUM) {
+      switch (c->nf) {
+        case 1: rc = project<VT_SUM, 1, true>(c, d_start, d_end, n, d_out, nout, d_counts, st, pool); break;
+        default: rc = project<VT_SUM, 0, true>(c, d_start, d_end, n, d_out, nout, d_counts, st, pool); break;
+      }
+    } else {
+      rc = project<VT_VOLUME, 4, true>(c, d_start, d_end, n, d_out, nout, d_counts, st, pool);
+    }
+    if (rc) return rc;
+    unsigned long long used = 0;
+    int over = 0;
+    HIP_OK(hipMemcpyAsync(&used, head, sizeof used, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&over, overflow, sizeof over, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    // Records fill whole chunks; on overflow the caller falls back to vt_adjoint.
+    *n_records = over ? (size_t)-1 : (size_t)std::min<unsigned long long>(used, capacity);
+    return VT_OK;
+  });
+}
+
 int vt_project_count(const vt_cloud *cc, const double *d_start, const double *d_end,
                      size_t nrays, int64_t *d_counts, void *stream) {
   return guarded([&]() -> int {
@@ -1399,18 +1530,7 @@ int vt_adjoint(const vt_cloud *cc, const double *d_start, const double *d_end, s
       ms = g_stats.kernel_ms;
     }
     const int64_t *cnt = d_counts ? d_counts : counts;
-    HIP_OK(hipMemsetAsync(offsets, 0, sizeof(int64_t), st));
-    {
-      size_t tmp = 0;
-      HIP_OK(rocprim::inclusive_scan(nullptr, tmp, cnt, offsets + 1, nrays,
-                                     rocprim::plus<int64_t>(), st));
-      void *t = B.get<char>(tmp);
-      HIP_OK(rocprim::inclusive_scan(t, tmp, cnt, offsets + 1, nrays,
-                                     rocprim::plus<int64_t>(), st));
-    }
-    int64_t nseg = 0;
-    HIP_OK(hipMemcpyAsync(&nseg, offsets + nrays, sizeof nseg, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
+    const int64_t nseg = scan_counts(cnt, nrays, offsets, B, st);
     if (nseg > (int64_t)0xFFFFFFFFll)
       return fail(VT_EINVAL, "more than 2^32 - 1 segments in one adjoint call");
     // 2. segment lists (slot, ds), ray-major
@@ -1435,51 +1555,59 @@ int vt_adjoint(const vt_cloud *cc, const double *d_start, const double *d_end, s
     ms += g_stats.kernel_ms;
     const int64_t segments = g_stats.segments;
     if (nseg == 0) return VT_OK;
-    Timer tm;
-    HIP_OK(hipEventRecord(tm.e0, st));
-    // 3. contributions per segment
-    const int ncf = mode == VT_VOLUME ? 4 : nf;
-    double *contrib = B.get<double>((size_t)nseg * ncf);
-    const int threads = 256;
-    const int rblocks = (int)((nrays + threads - 1) / threads);
-    if (mode == VT_SUM)
-      k_contrib_sum<<<rblocks, threads, 0, st>>>(n, offsets, ds, d_grad_out, nf, contrib);
-    else
-      k_contrib_vol<<<rblocks, threads, 0, st>>>(n, offsets, slot, ds, c->d_fields, d_grad_out,
-                                                 contrib);
-    HIP_OK(hipGetLastError());
-    // 4. stable sort of the segment indices by slot
-    uint32_t *idx = B.get<uint32_t>(nseg);
-    uint32_t *keys = B.get<uint32_t>(nseg);
-    uint32_t *perm = B.get<uint32_t>(nseg);
-    const int sblocks = (int)((nseg + threads - 1) / threads);
-    k_iota<<<sblocks, threads, 0, st>>>(nseg, idx);
-    HIP_OK(hipGetLastError());
-    {
-      const int nb = bits_for(c->n);
-      size_t tmp = 0;
-      HIP_OK(rocprim::radix_sort_pairs(nullptr, tmp, slot, keys, idx, perm, (size_t)nseg, 0, nb,
-                                       st));
-      void *t = B.get<char>(tmp);
-      HIP_OK(rocprim::radix_sort_pairs(t, tmp, slot, keys, idx, perm, (size_t)nseg, 0, nb, st));
-    }
-    // 5. per-cell sums in ray-major order
-    uint32_t *rs = B.get<uint32_t>(c->n);
-    uint32_t *re = B.get<uint32_t>(c->n);
-    HIP_OK(hipMemsetAsync(rs, 0, c->n * sizeof(uint32_t), st));
-    HIP_OK(hipMemsetAsync(re, 0, c->n * sizeof(uint32_t), st));
-    k_run_bounds<<<sblocks, threads, 0, st>>>(nseg, keys, rs, re);
-    HIP_OK(hipGetLastError());
-    const int cblocks = (int)((c->n + threads - 1) / threads);
-    k_reduce_cells<<<cblocks, threads, 0, st>>>((uint32_t)c->n, ncf, rs, re, perm, contrib,
-                                                 c->d_slot_id, d_grad_fields);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipEventRecord(tm.e1, st));
-    HIP_OK(hipStreamSynchronize(st));
-    float ms2 = 0;
-    HIP_OK(hipEventElapsedTime(&ms2, tm.e0, tm.e1));
+    const float ms2 = adjoint_tail(c, mode, n, offsets, slot, ds, nseg, d_grad_out,
+                                   d_grad_fields, B, st);
     g_stats.kernel_ms = ms + ms2;
     g_stats.segments = segments;
+    return VT_OK;
+  });
+}
+
+int vt_adjoint_recorded(const vt_cloud *cc, size_t nrays, int mode, const int64_t *d_counts,
+                        const void *d_records, size_t n_records, const double *d_grad_out,
+                        double *d_grad_fields, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (!rc) rc = check_mode(cc, mode);
+    if (rc) return rc;
+    if (mode != VT_SUM && mode != VT_VOLUME)
+      return fail(VT_EINVAL, "the adjoint is defined for reductions 'sum' and 'volume'");
+    if (!d_counts || (!d_records && n_records))
+      return fail(VT_EINVAL, "null argument");
+    if (n_records == (size_t)-1)
+      return fail(VT_EINVAL, "the recorded projection overflowed its record pool");
+    if (reinterpret_cast<uintptr_t>(d_records) % alignof(SegRec))
+      return fail(VT_EINVAL, "the record pool must be 16-byte aligned");
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_OK(hipMemsetAsync(d_grad_fields, 0, c->n * c->nf * sizeof(double), st));
+    g_stats = vt_call_stats{};
+    if (nrays == 0) return VT_OK;
+    const int64_t n = (int64_t)nrays;
+    DevBufs B;
+    int64_t *offsets = B.get<int64_t>(nrays + 1);
+    const int64_t nseg = scan_counts(d_counts, nrays, offsets, B, st);
+    if (nseg > (int64_t)0xFFFFFFFFll)
+      return fail(VT_EINVAL, "more than 2^32 - 1 segments in one adjoint call");
+    if ((size_t)nseg > n_records)
+      return fail(VT_EINVAL, "the records do not hold every segment the counts name");
+    if (nseg == 0) return VT_OK;
+    uint32_t *slot = B.get<uint32_t>(nseg);
+    double *ds = B.get<double>(nseg);
+    Timer tm;
+    HIP_OK(hipEventRecord(tm.e0, st));
+    const int threads = 256;
+    k_scatter_records<<<(int)((n_records + threads - 1) / threads), threads, 0, st>>>(
+        (uint64_t)n_records, static_cast<const SegRec *>(d_records), n, offsets, d_counts, slot,
+        ds);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(tm.e1, st));
+    const float ms2 = adjoint_tail(c, mode, n, offsets, slot, ds, nseg, d_grad_out,
+                                   d_grad_fields, B, st);
+    float ms1 = 0;
+    HIP_OK(hipEventElapsedTime(&ms1, tm.e0, tm.e1));
+    g_stats.kernel_ms = ms1 + ms2;
+    g_stats.segments = nseg;
     return VT_OK;
   });
 }
