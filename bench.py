@@ -6,6 +6,13 @@ Default workload (one step = one full image):
 Inputs (tree, fields, rays) are resident on the device before the timed
 region; tree build and host->device copies are reported separately.
 
+Other workloads (--workload): uniform100k (100k uniform points, 128^2),
+clustered2m_volume (volume rendering at 1024^2), multiview4m (4M points, 64
+cameras of random direction, extent and size in ONE batched launch), fit8m
+(8M points, 16 views at 512^2: forward + adjoint of a squared loss, with a
+central-difference check on 64 cells) and fit1m_volume (the volume adjoint on
+a 1M mesh at 256^2, with a central-difference check).
+
 Prints ONE JSON line.  Extra keys: rays_per_s, kernel_ms, fwd_adjoint (the
 same rays, forward plus the adjoint of a squared loss through torch.autograd),
 roofline (HBM bound,
@@ -42,6 +49,21 @@ WORKLOADS = {
     "clustered2m_volume": dict(desc="2M clustered periodic mesh (seed 2), RGBA transfer function, "
                     "1024^2 z-rays over [0,100], volume rendering",
                n=2_000_000, npix=1024, periodic=True, mode="volume"),
+    "multiview4m": dict(desc="4M clustered periodic mesh (seed 4), 64 cameras (seed 40): "
+                    "isotropic directions through the box centre, extents U[10,100], "
+                    "npix from {64,128,256,512,1024}; one batched launch, sum",
+               n=4_000_000, seed=4, periodic=True, mode="sum", kind="views", views=64,
+               cam_seed=40, sizes=(64, 128, 256, 512, 1024), extent=(10.0, 100.0)),
+    "fit8m": dict(desc="8M clustered periodic mesh (seed 5), 16 views at 512^2 (cameras seed "
+                  "50, extent 100); L = sum (proj(rho) - target)^2, target = "
+                  "proj(rho exp(0.1 N(0,1))) (seed 55); forward + adjoint by torch.autograd",
+               n=8_000_000, seed=5, periodic=True, mode="sum", kind="fit", views=16,
+               cam_seed=50, sizes=(512,), extent=(100.0, 100.0)),
+    "fit1m_volume": dict(desc="1M clustered periodic mesh (seed 6), RGBA transfer function, "
+                         "256^2 z-rays over [0,100]; L = sum (proj(rgba) - target)^2, target "
+                         "with alpha scaled by exp(0.1 N(0,1)) (seed 55); volume forward + "
+                         "adjoint by torch.autograd",
+               n=1_000_000, seed=6, npix=256, periodic=True, mode="volume", kind="fit"),
 }
 
 
@@ -50,17 +72,23 @@ def log(msg):
 
 
 def make_workload(name):
+    """Mesh, fields and views [(start, end), ...] of a workload."""
     from vortrace_amd import grid, synthetic
     w = WORKLOADS[name]
     t0 = time.perf_counter()
     if name == "uniform100k":
         pos, fields = synthetic.uniform_mesh(w["n"], seed=1)
     else:
-        pos, fields = synthetic.clustered_mesh(w["n"], seed=2)
+        pos, fields = synthetic.clustered_mesh(w["n"], seed=w.get("seed", 2))
     log(f"[bench] mesh {name}: {len(pos)} points in {time.perf_counter() - t0:.1f}s "
         f"(sha {synthetic.digest(pos, fields)})")
-    s, e = grid.generate_projection_grid([0.0, 100.0], w["npix"], [0.0, 100.0], None)
-    return pos, fields, s.reshape(-1, 3), e.reshape(-1, 3)
+    if "views" in w:
+        views = synthetic.camera_rays(w["cam_seed"], w["views"], sizes=w["sizes"],
+                                      extent_range=w["extent"])
+    else:
+        s, e = grid.generate_projection_grid([0.0, 100.0], w["npix"], [0.0, 100.0], None)
+        views = [(s.reshape(-1, 3), e.reshape(-1, 3))]
+    return pos, fields, views
 
 
 def algorithmic_bytes(R, C, n_points, n_f, n_out):
@@ -116,27 +144,40 @@ def cpu_baseline(pos, fields, start, end, periodic, mode, frac, gpu_vals):
             "values_within_1e-12": close}
 
 
-def fwd_adjoint(pc, rho, d_start, d_end, args, C, R):
+def fit_target(pc, fields, d_start, d_end, mode):
+    """Target image of the fit: proj(rho * exp(0.1 N(0,1) per cell)) for sum;
+    for volume the alpha column scaled by exp(0.1 N(0,1)) (RGBA stays valid).
+    Noise seed 55."""
+    from vortrace_amd import synthetic
+    z = synthetic.normal(np.random.default_rng(55), len(fields))
+    if mode == "sum":
+        noisy = fields * np.exp(0.1 * z)
+    else:
+        noisy = fields.copy()
+        noisy[:, 3] *= np.exp(0.1 * z)
+    pc.set_fields(noisy)
+    target = pc.projection_tensor(d_start, d_end, reduction=mode).clone()
+    pc.set_fields(fields)
+    return target[:, 0].contiguous() if mode == "sum" else target
+
+
+def fwd_adjoint(pc, fields, d_start, d_end, args, C, R, mode="sum"):
     """Forward plus adjoint on the same rays: one step evaluates
-    L = sum over pixels of (proj(rho) - target)^2 and dL/drho through
-    torch.autograd (projection kernel forward, adjoint kernel backward), with
-    target = proj(rho * exp(0.1 N(0,1) per cell))."""
+    L = sum over pixels of (proj(fields) - target)^2 and dL/dfields through
+    torch.autograd (projection kernel forward, adjoint kernels backward).
+    Returns (report, gradient, target)."""
     import torch
     import vortrace_amd as vt
-    from vortrace_amd import synthetic
     dev = d_start.device
-    rng = np.random.default_rng(55)
-    noisy = rho * np.exp(0.1 * synthetic.normal(rng, len(rho)))
-    pc.set_fields(noisy)
-    target = pc.projection_tensor(d_start, d_end)[:, 0].clone()
-    rho_t = torch.from_numpy(np.ascontiguousarray(rho, dtype=np.float64)).to(dev)
+    target = fit_target(pc, fields, d_start, d_end, mode)
+    f_t = torch.from_numpy(np.ascontiguousarray(fields, dtype=np.float64)).to(dev)
 
     split = {"forward_ms": [], "backward_ms": []}
 
     def step():
-        f = rho_t.clone().requires_grad_(True)
+        f = f_t.clone().requires_grad_(True)
         t0 = time.perf_counter()
-        p = vt.autograd.project(pc, f, d_start, d_end, "sum")
+        p = vt.autograd.project(pc, f, d_start, d_end, mode)
         loss = ((p - target) ** 2).sum()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -144,7 +185,7 @@ def fwd_adjoint(pc, rho, d_start, d_end, args, C, R):
         torch.cuda.synchronize()
         split["forward_ms"].append(1e3 * (t1 - t0))
         split["backward_ms"].append(1e3 * (time.perf_counter() - t1))
-        return f.grad, pc._adjoint_stats
+        return f.grad, pc._forward_stats, pc._adjoint_stats
 
     for i in range(args.warmup):
         step()
@@ -153,24 +194,104 @@ def fwd_adjoint(pc, rho, d_start, d_end, args, C, R):
     split["backward_ms"].clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    kms = []
+    fms, kms = [], []
+    g = None
     for i in range(args.steps):
-        g, st = step()
-        kms.append(st.kernel_ms)
+        g, fst, ast = step()
+        fms.append(fst.kernel_ms)
+        kms.append(ast.kernel_ms)
     torch.cuda.synchronize()
     ms = 1e3 * (time.perf_counter() - t0) / args.steps
-    nz = int(torch.count_nonzero(g).item())
+    nz = int(torch.count_nonzero(g.reshape(len(fields), -1).abs().amax(dim=1)).item())
     fwd_ms, bwd_ms = float(np.mean(split["forward_ms"])), float(np.mean(split["backward_ms"]))
     log(f"[bench] fwd+adjoint {args.steps} steps: {ms:.2f} ms/step (forward {fwd_ms:.2f}, "
-        f"backward {bwd_ms:.2f}), adjoint kernels {float(np.mean(kms)):.2f} ms, {nz} cells "
-        f"with a gradient")
-    pc.set_fields(rho)
-    return {"ms_per_step": ms, "crossings_per_s": C / (ms * 1e-3), "rays_per_s": R / (ms * 1e-3),
-            "adjoint_kernel_ms": float(np.mean(kms)), "forward_ms": fwd_ms,
-            "backward_ms": bwd_ms, "cells_with_gradient": nz,
-            "step": "L = sum (proj(rho) - target)^2 and dL/drho by torch.autograd "
-                    "(forward kernel recording its segments + adjoint kernels over the "
-                    "records), FP64"}
+        f"backward {bwd_ms:.2f}), forward kernels {float(np.mean(fms)):.2f} ms, adjoint "
+        f"kernels {float(np.mean(kms)):.2f} ms, {nz} cells with a gradient")
+    rep = {"ms_per_step": ms, "crossings_per_s": C / (ms * 1e-3), "rays_per_s": R / (ms * 1e-3),
+           "forward_kernel_ms": float(np.mean(fms)),
+           "adjoint_kernel_ms": float(np.mean(kms)), "forward_ms": fwd_ms,
+           "backward_ms": bwd_ms, "cells_with_gradient": nz,
+           "step": "L = sum (proj(fields) - target)^2 and dL/dfields by torch.autograd "
+                   "(forward kernel recording its segments + adjoint kernels over the "
+                   "records), FP64"}
+    return rep, g, target
+
+
+def fd_check(pc, fields, grad, target, d_start, d_end, mode, ncheck=64, seed=56):
+    """Central differences of the fit loss on `ncheck` random cells with a
+    nonzero gradient (one random channel each for RGBA).  L changes only
+    through the rays that cross the cell, and is evaluated as
+    L(+h) - L(-h) = sum over those rays of (p+ - p-)(p+ + p- - 2 t), which
+    avoids the cancellation of the whole image's sum.  Steps: 1e-3 relative
+    for sum and for R, G, B (L is quadratic in each of them, so the difference
+    is exact up to rounding) and 1e-4 relative for alpha (truncation
+    <= (h ds)^2 / 6 relative).  Each cell must satisfy
+    |fd - g| <= 1e-6 max(|fd|, |g|) + E, with E the worst-case rounding of the
+    difference: sum over its rays of 2 n eps (|p+| + |p-|) |p+ + p- - 2t| / 2h
+    (n = the ray's segments)."""
+    import torch
+    import vortrace_amd as vt
+    dev = d_start.device
+    rng = np.random.default_rng(seed)
+    g = grad.detach().reshape(len(fields), -1)
+    nf = g.shape[1]
+    f0 = torch.from_numpy(np.ascontiguousarray(fields, dtype=np.float64)).to(dev)
+    f0 = f0.reshape(len(fields), -1)
+    cand = torch.nonzero(g.abs().amax(dim=1) > 0).flatten().cpu().numpy()
+    order = np.argsort(rng.random(len(cand)), kind="stable")
+    picks = []
+    for c in cand[order]:
+        ch = min(int(nf * rng.random()), nf - 1)
+        v = float(f0[c, ch])
+        h = (1e-4 if mode == "volume" and ch == 3 else 1e-3) * abs(v)
+        if h == 0.0 or (mode == "volume" and ch < 3 and (v - h < 0.0 or v + h > 1.0)):
+            continue
+        picks.append((int(c), ch, v, h))
+        if len(picks) == ncheck:
+            break
+    mark = torch.zeros(len(fields), dtype=torch.bool, device=dev)
+    mark[torch.tensor([p[0] for p in picks], dtype=torch.int64, device=dev)] = True
+    _, ids, _, _, off = pc.segments_tensor(d_start, d_end, reduction=mode)
+    hit = torch.nonzero(mark[ids]).flatten()
+    hit_cell = ids[hit]
+    hit_ray = torch.searchsorted(off, hit, right=True) - 1
+    nseg = off[1:] - off[:-1]
+    del ids, mark
+    tgt = target.reshape(target.shape[0], -1)
+    eps = float(np.finfo(np.float64).eps)
+    worst, strict, ok, nrays = 0.0, 0, True, []
+    for c, ch, v, h in picks:
+        rays = torch.unique(hit_ray[hit_cell == c])
+        nrays.append(int(rays.numel()))
+        s_sub, e_sub, t_sub = d_start[rays].contiguous(), d_end[rays].contiguous(), tgt[rays]
+        n_sub = nseg[rays].to(torch.float64).reshape(-1, 1)
+
+        def proj(val):
+            f = f0.clone()
+            f[c, ch] = val
+            pc.set_fields(f.reshape(fields.shape))
+            return pc.projection_tensor(s_sub, e_sub, reduction=mode)
+
+        pp, pm = proj(v + h), proj(v - h)
+        step = (v + h) - (v - h)
+        w = (pp + pm - 2.0 * t_sub)
+        fd = float(((pp - pm) * w).sum()) / step
+        rnd = float((2.0 * n_sub * eps * (pp.abs() + pm.abs()) * w.abs()).sum()) / step
+        gv = float(g[c, ch])
+        err = abs(fd - gv)
+        rel = err / max(abs(fd), abs(gv), 1e-300)
+        strict += rel <= 1e-6
+        ok &= err <= 1e-6 * max(abs(fd), abs(gv)) + rnd
+        worst = max(worst, rel)
+    pc.set_fields(fields)
+    log(f"[bench] central differences on {len(picks)} cells ({min(nrays)}-{max(nrays)} rays "
+        f"each): {strict} within 1e-6 relative (max {worst:.2e}); all within 1e-6 + "
+        f"rounding bound: {ok}")
+    return {"cells": len(picks), "within_1e-6": int(strict), "max_rel_err": worst,
+            "all_within_bound": bool(ok),
+            "step": "1e-3 relative" if mode == "sum" else "1e-3 relative (RGB), 1e-4 (alpha)",
+            "bound": "1e-6 max(|fd|,|g|) + worst-case rounding of the difference",
+            "rays_per_cell": [min(nrays), max(nrays)]}
 
 
 def main():
@@ -179,8 +300,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="clustered2m", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-sample", type=float, default=0.25,
-                    help="fraction of the rays timed on the CPU reference")
+    ap.add_argument("--cpu-sample", type=float, default=None,
+                    help="fraction of the rays timed on the CPU reference (default: a "
+                         "quarter, or about 2.5e8 crossings if fewer)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="multi-view workload: keep the current launch parameters")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint measurement")
@@ -210,9 +334,10 @@ def main():
     from vortrace_amd import _lib, synthetic
 
     w = WORKLOADS[args.workload]
-    pos, fields, start, end = make_workload(args.workload)
-    if w["mode"] == "volume":
-        # kappa: median ray opacity 0.5 on a 64^2 probe of the same view
+    kind, mode = w.get("kind", "image"), w["mode"]
+    pos, fields, views = make_workload(args.workload)
+    if mode == "volume":
+        # kappa: median ray opacity 0.5 on a 64^2 z-probe of the box
         pc_probe = vt.ProjectionCloud(pos, fields, boundbox=[0, 100, 0, 100, 0, 100],
                                       periodic=True)
         ps, pe = vt.grid.generate_projection_grid([0.0, 100.0], 64, [0.0, 100.0], None)
@@ -232,80 +357,112 @@ def main():
     info = pc._cloud.tree_info()
     dev = torch.device("cuda", 0)
     t0 = time.perf_counter()
-    d_start = torch.from_numpy(start).to(dev)
-    d_end = torch.from_numpy(end).to(dev)
+    d_views = [(torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)) for s, e in views]
+    if len(d_views) == 1:
+        d_start, d_end = d_views[0]
+    else:
+        d_start = torch.cat([v[0] for v in d_views])
+        d_end = torch.cat([v[1] for v in d_views])
     torch.cuda.synchronize()
     t_h2d = time.perf_counter() - t0
+    start = np.concatenate([v[0] for v in views]) if len(views) > 1 else views[0][0]
+    end = np.concatenate([v[1] for v in views]) if len(views) > 1 else views[0][1]
     R = len(start)
-    n_f = 4 if w["mode"] == "volume" else 1
-    n_out = 3 if w["mode"] == "volume" else 1
+    n_f = 4 if mode == "volume" else 1
+    n_out = 3 if mode == "volume" else 1
     log(f"[bench] tree: {info['nodes']} nodes, depth {info['max_depth']}, host build "
-        f"{info['build_seconds']:.2f}s (cloud+upload {t_build:.2f}s); rays H2D {t_h2d:.3f}s")
+        f"{info['build_seconds']:.2f}s (cloud+upload {t_build:.2f}s); {len(views)} view(s), "
+        f"{R} rays, H2D {t_h2d:.3f}s")
+
+    tuning = None
+    if kind == "views" and not args.no_autotune:
+        tuning = vt.tuning.autotune(pc, d_start, d_end, reduction=mode, stride=16,
+                                    walk_batches=(24, 32, 48), leaf_batches=(16, 24, 32),
+                                    ray_parts=(8,), repeats=1)
+        log(f"[bench] autotune on every 16th ray: {tuning['best']} "
+            f"({tuning['kernel_ms']:.2f} ms)")
+
+    def forward():
+        if kind == "views":  # one launch over every view's rays
+            return pc.batch_projection_tensor(d_views, reduction=mode)
+        return pc.projection_tensor(d_start, d_end, reduction=mode)
 
     out = None
     for i in range(args.warmup):
-        out = pc.projection_tensor(d_start, d_end, reduction=w["mode"])
+        out = forward()
         log(f"[bench] warmup {i + 1}/{args.warmup} done")
     torch.cuda.synchronize()
     kms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        out = pc.projection_tensor(d_start, d_end, reduction=w["mode"])
+        out = forward()
         kms.append(_lib.last_call_stats().kernel_ms)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     st = _lib.last_call_stats()
+    if isinstance(out, list):
+        out = torch.cat(out)
     C = int(st.segments)
     ms_per_step = 1e3 * wall / args.steps
     kernel_ms = float(np.mean(kms))
-    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, kernel {kernel_ms:.2f} ms "
-        f"(min {min(kms):.2f}), {C} crossings, {st.queries} kNN queries, "
+    log(f"[bench] forward {args.steps} steps: {ms_per_step:.2f} ms/step, kernel "
+        f"{kernel_ms:.2f} ms (min {min(kms):.2f}), {C} crossings, {st.queries} kNN queries, "
         f"{st.slow_queries} K_EXPAND, walk depth {st.max_walk_depth}, "
         f"kNN stack {st.max_knn_depth}")
+    fwd = {"ms_per_step": ms_per_step, "crossings_per_s": C / (ms_per_step * 1e-3),
+           "rays_per_s": R / (ms_per_step * 1e-3), "kernel_ms": kernel_ms}
 
     fa = None
-    if w["mode"] == "sum" and not args.no_adjoint:
-        fa = fwd_adjoint(pc, fields, d_start, d_end, args, C, R)
+    if kind == "fit" or (mode == "sum" and not args.no_adjoint):
+        fa, grad, target = fwd_adjoint(pc, fields, d_start, d_end, args, C, R, mode)
+        if kind == "fit":
+            fa["fd_check"] = fd_check(pc, fields, grad, target, d_start, d_end, mode)
+        del grad, target
 
     bytes_alg = algorithmic_bytes(R, C, len(pos), n_f, n_out)
     traffic, l2_hit = read_traffic(args.workload)
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
+    step = fa if kind == "fit" else fwd
     result = {
         "metric": METRIC,
-        "value": C / (ms_per_step * 1e-3),
+        "value": step["crossings_per_s"],
         "unit": "cell-crossings/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": step["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": w["desc"], "n_points": len(pos), "rays": R,
+        "config": {"workload": w["desc"], "step": "forward + adjoint" if kind == "fit"
+                   else "forward", "n_points": len(pos), "views": len(views), "rays": R,
                    "crossings_per_step": C, "periodic": w["periodic"],
-                   "reduction": w["mode"], "tree_depth": int(info["max_depth"])},
-        "rays_per_s": R / (ms_per_step * 1e-3),
+                   "reduction": mode, "tree_depth": int(info["max_depth"])},
+        "rays_per_s": step["rays_per_s"],
         "kernel_ms": kernel_ms,
         "kernel_crossings_per_s": C / (kernel_ms * 1e-3),
         "knn_queries_per_step": int(st.queries),
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+        "forward": fwd,
+        "roofline": {"bound": "hbm", "kernel": "forward projection", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
                      "hbm_gbs_measured": (traffic / (kernel_ms * 1e-3) / 1e9
                                           if traffic else None),
                      "l2_hit_rate": l2_hit,
                      "algorithmic_bytes_per_launch": bytes_alg},
         "fwd_adjoint": fa,
+        "tuning": tuning,
         "setup_seconds": {"tree_build_host": info["build_seconds"],
                           "cloud_and_upload": t_build, "rays_h2d": t_h2d},
         "cpu_baseline": None,
     }
     if not args.no_cpu:
+        frac = args.cpu_sample if args.cpu_sample else min(0.25, 2.5e8 / max(C, 1))
         gv = out.cpu().numpy().reshape(R, -1)
         result["cpu_baseline"] = cpu_baseline(pos, fields, start, end, w["periodic"],
-                                              w["mode"], args.cpu_sample, gv)
+                                              mode, frac, gv)
     print(json.dumps(result), flush=True)
 
 

@@ -60,6 +60,7 @@ class ProjectFields(torch.autograd.Function):
                 records = None  # pool too small: the backward walks the rays
         else:
             out = proj.project_tensor(pc._cloud, mode, counts=counts)
+        pc._forward_stats = _lib.last_call_stats()
         if ctx.needs_input_grad[0]:
             pc._segment_estimate = int(counts.sum().item())
         ctx.save_for_backward(fields, counts)
