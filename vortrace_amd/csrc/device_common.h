@@ -78,6 +78,7 @@ constexpr int kParts = 8;
 // Everything a kernel needs to know about a built cloud (passed by value).
 struct DevCloud {
   const DNode *nodes;    // inner nodes, pre-order
+  const DRec *recs;      // pair records (ray engine searches)
   const Gen *gen;        // slot order, followed by kLeafMaxSize padding records
   const double *fields;  // slot order, nf per slot
   double rb_lo[3], rb_hi[3];  // nanoflann root_bbox
@@ -86,7 +87,8 @@ struct DevCloud {
   double inv_sqrt2;           // 1.0 / std::sqrt(2.0), computed on the host
   uint32_t n, nf;
   int periodic;
-  uint32_t root;  // reference of the root (a leaf when n <= kLeafMaxSize)
+  uint32_t root;   // reference of the root (a leaf when n <= kLeafMaxSize)
+  uint32_t rroot;  // the root among pair records
 };
 
 // Per-lane scratch that does not fit in registers/LDS (device memory).
@@ -166,6 +168,44 @@ __device__ __forceinline__ DNodeV decode_node(const double2 v0, const double2 v1
 __device__ __forceinline__ DNodeV load_node(const DNode *nodes, uint32_t i) {
   const double2 *p = reinterpret_cast<const double2 *>(nodes + i);
   return decode_node(gld(p), gld(p + 1));
+}
+
+// A pair record, read at once, with the node's own fields decoded.
+struct DRecV {
+  DNodeV x;                    // the record's node
+  double lo1, hi1, lo2, hi2;   // its children's bounds
+  uint32_t c11, c12, c21, c22;  // its children's references, with dim bits
+};
+
+__device__ __forceinline__ DRecV load_rec(const DRec *recs, uint32_t r) {
+  const double2 *p = reinterpret_cast<const double2 *>(recs + r);
+  const double2 v0 = gld(p), v1 = gld(p + 1), v2 = gld(p + 2), v3 = gld(p + 3);
+  const uint4 v4 = gld(reinterpret_cast<const uint4 *>(p + 4));
+  DRecV R;
+  R.x = decode_node(v0, v1);
+  R.lo1 = v2.x;
+  R.hi1 = v2.y;
+  R.lo2 = v3.x;
+  R.hi2 = v3.y;
+  R.c11 = v4.x;
+  R.c12 = v4.y;
+  R.c21 = v4.z;
+  R.c22 = v4.w;
+  return R;
+}
+
+// Child b (0: child1, 1: child2) of a record's node, as a node: its bounds
+// and children from the record, and the parent's bound facing it as pb.
+__device__ __forceinline__ DNodeV rec_child(const DRecV &R, uint32_t b) {
+  DNodeV n;
+  n.lo = b ? R.lo2 : R.lo1;
+  n.hi = b ? R.hi2 : R.hi1;
+  const uint32_t a1 = b ? R.c21 : R.c11, a2 = b ? R.c22 : R.c12;
+  n.dim = ((a1 >> 30) & 1u) | ((a2 >> 29) & 2u);
+  n.c1 = a1 & ~kDimBit;
+  n.c2 = a2 & ~kDimBit;
+  n.pb = b ? R.x.hi : R.x.lo;
+  return n;
 }
 
 // Stack tag of a child reference taken as a far child at a node cutting

@@ -82,7 +82,7 @@ struct Search {
     }
     st.reset();
     ptop = INFINITY;
-    cur = C.root;
+    cur = C.rroot;
     mode = (cur & kRefLeaf) ? kTmLeaf : kTmNode;
     depth = 0;
     lost_min = DBL_MAX;
@@ -111,18 +111,10 @@ struct Search {
     dd[2] = f == 2 ? v : dd[2];
   }
 
-  // One inner node (modes kTmNode / kTmEnter): choose the child to visit
-  // first and push the other.
-  __device__ __forceinline__ void node_step(const DevCloud &C, KStack &st) {
-    const DNodeV nd = load_node(C.nodes, cur);
-    if (mode == kTmEnter) {
-      // Entering the far child of a popped entry: dists[idx] = cut_dist,
-      // mindistsq from the entry; the old dists[idx] is restored on return.
-      const double c = qd(pdim) - nd.pb;
-      st.push(kUndo | pdim, ddd(pdim), ptop);
-      set_dd(pdim, c * c);
-      mind = pv;
-    }
+  // One level of searchLevel at inner node nd: choose the child to visit
+  // first and push the other (or, on the first descent, put it in the LDS
+  // ring).  Returns the child to visit.
+  __device__ __forceinline__ uint32_t level(KStack &st, const DNodeV &nd) {
     const uint32_t f = nd.dim;
     double cut;
     const bool low_first = choose(nd, qd(f), cut);
@@ -146,7 +138,26 @@ struct Search {
       st.push(tag, m2, ptop);
       ptop = fminf(ptop, bound_below(m2));
     }
-    const uint32_t near = low_first ? nd.c1 : nd.c2;
+    return low_first ? nd.c1 : nd.c2;
+  }
+
+  // One step at inner node `cur` (modes kTmNode / kTmEnter): one pair record
+  // is read; an even-depth node and its near child are searched from it (two
+  // levels), an odd-depth node (entered off the stack) on its own.
+  __device__ __forceinline__ void node_step(const DevCloud &C, KStack &st) {
+    const bool odd = (cur & kRefOdd) != 0u;
+    const DRecV R = load_rec(C.recs, odd ? (cur & (kRefOdd - 1u)) >> 1 : cur);
+    const DNodeV n0 = odd ? rec_child(R, cur & 1u) : R.x;
+    if (mode == kTmEnter) {
+      // Entering the far child of a popped entry: dists[idx] = cut_dist,
+      // mindistsq from the entry; the old dists[idx] is restored on return.
+      const double c = qd(pdim) - n0.pb;
+      st.push(kUndo | pdim, ddd(pdim), ptop);
+      set_dd(pdim, c * c);
+      mind = pv;
+    }
+    uint32_t near = level(st, n0);
+    if (!odd && !(near & kRefLeaf)) near = level(st, rec_child(R, near & 1u));
     cur = near;
     mode = kTmNode;
     if (near & kRefLeaf) {
@@ -251,7 +262,7 @@ struct Search {
           // the same path again, pushing the far children that can.
           st.reset();
           ptop = INFINITY;
-          cur = C.root;
+          cur = C.rroot;
           mode = kTmNode;
         }
         depth = -1;

@@ -60,6 +60,28 @@ struct alignas(16) DNode {
 };
 static_assert(sizeof(DNode) == 32, "device node must be 32 bytes");
 
+// Pair record of the ray engine's searches: an inner node X at even depth
+// (the root has depth 0) together with the split data of its two children, so
+// one 80-byte read serves two levels of a descent.  Inner nodes at odd depth
+// have no record of their own: they are read from their parent's.
+// References in pair records and in the engine's stack tags:
+//   even inner node:  record index r                         (r < 2^27)
+//   odd inner node:   kRefOdd | r << 1 | b, child b (0: child1, 1: child2)
+//                     of the node of record r
+//   leaf:             kRefLeaf | first slot
+// with bit 30 of the c* fields carrying cut-dimension bits as in DNode.
+constexpr uint32_t kRefOdd = 1u << 28;
+constexpr uint32_t kMaxRecs = 1u << 27;
+struct alignas(16) DRec {
+  double lo, hi;     // X: divlow, divhigh
+  uint32_t c1, c2;   // X: child references | (dim & 1) << 30, | (dim >> 1) << 30
+  double pb;         // X: as DNode::pb (entering X as its parent's far child)
+  double lo1, hi1;   // child1 of X when inner: divlow, divhigh
+  double lo2, hi2;   // child2 of X when inner
+  uint32_t c11, c12, c21, c22;  // children of child1 / child2 (with dim bits)
+};
+static_assert(sizeof(DRec) == 80, "pair record must be 80 bytes");
+
 struct alignas(16) Gen {  // one generator, slot order
   double x, y, z;
   uint32_t start, end;    // slot range of the generator's leaf (kNone-like

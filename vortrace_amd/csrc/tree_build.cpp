@@ -242,6 +242,54 @@ void make_device_tree(const HostTree &t, DeviceTree &out) {
     d.c2 = ref(nd.a) | ((nd.b & 2u) ? kDimBit : 0u);
   }
   out.root = ref(0);
+
+  // Pair records: depth parity, record index of every even inner node, and
+  // the record-relative reference of every node.
+  std::vector<uint8_t> odd(nh, 0);
+  std::vector<uint32_t> rec(nh, 0), parent(nh, 0);
+  uint32_t nrec = 0;
+  for (size_t h = 0; h < nh; ++h) {
+    const Node &nd = t.nodes[h];
+    if (nd.b & kLeafBit) continue;
+    odd[h + 1] = odd[nd.a] = (uint8_t)!odd[h];
+    parent[h + 1] = parent[nd.a] = (uint32_t)h;
+    if (!odd[h]) rec[h] = nrec++;
+  }
+  if (nrec >= kMaxRecs) throw std::invalid_argument("too many points for one cloud (2^27 pair records)");
+  auto rref = [&](size_t h) -> uint32_t {
+    const Node &c = t.nodes[h];
+    if (c.b & kLeafBit) return kRefLeaf | c.a;
+    if (!odd[h]) return rec[h];
+    const size_t p = parent[h];
+    return kRefOdd | rec[p] << 1 | (h == p + 1 ? 0u : 1u);
+  };
+  auto dims = [&](size_t h, uint32_t &a1, uint32_t &a2) {
+    const Node &nd = t.nodes[h];
+    a1 = rref(h + 1) | ((nd.b & 1u) ? kDimBit : 0u);
+    a2 = rref(nd.a) | ((nd.b & 2u) ? kDimBit : 0u);
+  };
+  out.recs.assign(nrec, DRec{});
+  for (size_t h = 0; h < nh; ++h) {
+    const Node &nd = t.nodes[h];
+    if ((nd.b & kLeafBit) || odd[h]) continue;
+    DRec &r = out.recs[rec[h]];
+    r.lo = nd.lo;
+    r.hi = nd.hi;
+    r.pb = nd.pb;
+    dims(h, r.c1, r.c2);
+    const size_t h1 = h + 1, h2 = nd.a;
+    if (!(t.nodes[h1].b & kLeafBit)) {
+      r.lo1 = t.nodes[h1].lo;
+      r.hi1 = t.nodes[h1].hi;
+      dims(h1, r.c11, r.c12);
+    }
+    if (!(t.nodes[h2].b & kLeafBit)) {
+      r.lo2 = t.nodes[h2].lo;
+      r.hi2 = t.nodes[h2].hi;
+      dims(h2, r.c21, r.c22);
+    }
+  }
+  out.rroot = nh ? rref(0) : 0u;
 }
 
 namespace {

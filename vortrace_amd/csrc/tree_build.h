@@ -27,8 +27,10 @@ void build_tree(const double *pts, size_t n, int threads, HostTree &out);
 // std::invalid_argument when the tree exceeds the reference encoding.
 struct DeviceTree {
   std::vector<DNode> nodes;
+  std::vector<DRec> recs;  // pair records of the even-depth inner nodes, pre-order
   std::vector<uint32_t> leaf_start, leaf_end;  // per slot: its leaf's slot range
-  uint32_t root = 0;
+  uint32_t root = 0;   // root reference among DNodes
+  uint32_t rroot = 0;  // root reference among pair records
 };
 void make_device_tree(const HostTree &t, DeviceTree &out);
 

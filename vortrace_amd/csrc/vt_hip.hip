@@ -564,6 +564,7 @@ struct vt_cloud {
   // device copy
   int device = -1;
   DNode *d_nodes = nullptr;
+  DRec *d_recs = nullptr;
   Gen *d_gen = nullptr;
   double *d_fields = nullptr;
   uint32_t *d_slot_id = nullptr;
@@ -577,7 +578,7 @@ struct vt_cloud {
   DevCloud dc{};
 
   void free_device() {
-    for (void *p : {(void *)d_nodes, (void *)d_gen, (void *)d_fields, (void *)d_slot_id,
+    for (void *p : {(void *)d_nodes, (void *)d_recs, (void *)d_gen, (void *)d_fields, (void *)d_slot_id,
                     (void *)d_err, (void *)d_stats, (void *)scratch.knn_spill,
                     (void *)scratch.walk_tag, (void *)scratch.walk_s,
                     (void *)scratch.big_id, (void *)scratch.big_d, (void *)d_defer})
@@ -585,6 +586,7 @@ struct vt_cloud {
     d_defer = nullptr;
     defer_cap = 0;
     d_nodes = nullptr;
+    d_recs = nullptr;
     d_gen = nullptr;
     d_fields = nullptr;
     d_slot_id = nullptr;
@@ -1110,6 +1112,7 @@ int vt_cloud_upload(vt_cloud *c) {
     c->tree_built = false;
     HIP_OK(hipGetDevice(&c->device));
     c->d_nodes = dev_alloc<DNode>(std::max<size_t>(1, dt.nodes.size()));
+    c->d_recs = dev_alloc<DRec>(std::max<size_t>(1, dt.recs.size()));
     // kLeafMaxSize zero records of padding: leaf steps load whole chunks.
     c->d_gen = dev_alloc<Gen>(c->n + kLeafMaxSize);
     c->d_fields = dev_alloc<double>(fslot.size());
@@ -1118,6 +1121,9 @@ int vt_cloud_upload(vt_cloud *c) {
     c->d_stats = dev_alloc<Stats>(1);
     if (!dt.nodes.empty())
       HIP_OK(hipMemcpy(c->d_nodes, dt.nodes.data(), dt.nodes.size() * sizeof(DNode),
+                       hipMemcpyHostToDevice));
+    if (!dt.recs.empty())
+      HIP_OK(hipMemcpy(c->d_recs, dt.recs.data(), dt.recs.size() * sizeof(DRec),
                        hipMemcpyHostToDevice));
     gen.resize(c->n + kLeafMaxSize, Gen{0.0, 0.0, 0.0, 0xFFFFFFFFu, 0xFFFFFFFFu});
     HIP_OK(hipMemcpy(c->d_gen, gen.data(), gen.size() * sizeof(Gen), hipMemcpyHostToDevice));
@@ -1129,6 +1135,7 @@ int vt_cloud_upload(vt_cloud *c) {
     c->kspill = (int)t.max_depth + 2;
     DevCloud &d = c->dc;
     d.nodes = c->d_nodes;
+    d.recs = c->d_recs;
     d.gen = c->d_gen;
     d.fields = c->d_fields;
     for (int k = 0; k < 3; ++k) {
@@ -1142,6 +1149,7 @@ int vt_cloud_upload(vt_cloud *c) {
     d.nf = (uint32_t)c->nf;
     d.periodic = c->periodic;
     d.root = dt.root;
+    d.rroot = dt.rroot;
     c->tree_built = true;
     return VT_OK;
   });
