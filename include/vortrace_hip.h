@@ -185,6 +185,29 @@ int vt_adjoint_recorded(const vt_cloud *cloud, size_t nrays, int mode, const int
                         const void *d_records, size_t n_records, const double *d_grad_out,
                         double *d_grad_fields, void *stream);
 
+/* grid.generate_projection_grid (vortrace/grid.py:100-192) on the device,
+ * for many views at once: view v's rays follow those of views 0..v-1, ray
+ * i * ny + j of a view (pixel x_i, y_j, numpy.linspace pixel centres of
+ * grid.py:35-44) starts at (x_i, y_j, bounds[0]) and ends at
+ * (x_i, y_j, bounds[1]), rotated about `center` by `rot` (row-major R,
+ * grid.py:86-97, computed by the caller) when rotate != 0.  bounds are
+ * those after the handedness flip of grid.py:163-172.  d_start / d_end hold
+ * sum(nx * ny) x 3 values.  The bits are numpy's for unrotated grids; with a
+ * rotation, numpy.dot's rows are evaluated as fma(p2, R_k2, fma(p1, R_k1,
+ * p0 * R_k0)), the order of OpenBLAS' x86-64 kernels behind numpy.  No
+ * reference counterpart as a C function: the reference builds rays in numpy. */
+typedef struct {
+  double extent[4]; /* xlo, xhi, ylo, yhi */
+  double bounds[2]; /* z of start and end points before the rotation */
+  double center[3];
+  double rot[9];
+  int64_t nx, ny;
+  int32_t rotate;
+  int32_t pad_;
+} vt_view;
+int vt_grid_rays(const vt_view *views, size_t nviews, double *d_start, double *d_end,
+                 void *stream);
+
 /* Launch parameters of the ray kernels (process-wide): lanes of a wave that
  * run their walk steps together, lanes that run their leaf steps together
  * (both 1..64), and ray partitions (1, or 8: one per XCD).  Results do not

@@ -48,8 +48,20 @@ def lib():
         L.vto_detailed_copy.argtypes = [P] * 6
         L.vto_detailed_destroy.argtypes = [P]
         L.vto_set_mutation.argtypes = [I]
+        L.vto_rotate_about.argtypes = [P, S, P, P, P]
         _lib = L
     return _lib
+
+
+def rotate_about(p, rot, center):
+    """(p - center) . rot.T + center in numpy.dot's evaluation order on
+    OpenBLAS' x86-64 kernels (vto_rotate_about)."""
+    p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1, 3)
+    R = np.ascontiguousarray(rot, dtype=np.float64)
+    c = np.ascontiguousarray(np.zeros(3) + np.asarray(center), dtype=np.float64)
+    out = np.empty_like(p)
+    lib().vto_rotate_about(_p(p), len(p), _p(R), _p(c), _p(out))
+    return out
 
 
 def set_mutation(m: int) -> None:

@@ -1091,3 +1091,20 @@ void vto_detailed_destroy(vto_segments *s) {
   free(s->segs);
   free(s);
 }
+
+/* ---------------------------------------------------------------------------
+ * Ray grids (vortrace/grid.py:100-192).  Rotated grids: start -= center;
+ * numpy.dot(start, R.T); start += center.  numpy hands the 3-term dot to
+ * OpenBLAS, whose x86-64 kernels evaluate row k as
+ * fma(p2, R[k][2], fma(p1, R[k][1], p0 * R[k][0])).  This restates that order
+ * (checked against numpy itself by the tests) so that a device generator can
+ * be checked against it.
+ * ------------------------------------------------------------------------- */
+void vto_rotate_about(const double *p, size_t n, const double *R, const double *c,
+                      double *out) {
+  for (size_t i = 0; i < n; ++i) {
+    const double a0 = p[3 * i] - c[0], a1 = p[3 * i + 1] - c[1], a2 = p[3 * i + 2] - c[2];
+    for (int k = 0; k < 3; ++k)
+      out[3 * i + k] = fma(a2, R[3 * k + 2], fma(a1, R[3 * k + 1], a0 * R[3 * k])) + c[k];
+  }
+}

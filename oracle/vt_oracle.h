@@ -61,6 +61,9 @@ void vto_detailed_copy(const vto_segments *s, double *values, int64_t *cell_ids,
                        double *s_enter, double *s_exit, int64_t *offsets);
 void vto_detailed_destroy(vto_segments *s);
 
+/* p (n x 3) rotated about c by R (row-major) in numpy.dot's evaluation order. */
+void vto_rotate_about(const double *p, size_t n, const double *R, const double *c, double *out);
+
 #ifdef __cplusplus
 }
 #endif

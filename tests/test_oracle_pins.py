@@ -316,6 +316,41 @@ def _pins_hold():
     return True
 
 
+ROTATED_VIEWS = [
+    dict(extent=[10.0, 90.0], nres=24, bounds=[0.0, 100.0], center=[50.0, 50.0, 50.0],
+         yaw=0.7, pitch=1.1, roll=0.3),
+    dict(extent=[[0.0, 100.0], [20.0, 60.0]], nres=(9, 13), bounds=[100.0, 0.0],
+         center=[50, 40, 60], yaw=2.5, pitch=0.2),
+    dict(extent=[-3.0, 7.0], nres=5, bounds=[-1.0, 4.0], center=[1.0, 2.0, 3.0], proj="yz"),
+    dict(extent=[0.0, 1.0], nres=(1, 3), bounds=[0.0, 1.0], center=[0.5, 0.5, 0.5],
+         proj="zy"),
+]
+
+
+@pytest.mark.parametrize("view", range(len(ROTATED_VIEWS)))
+def test_rotation_order_matches_numpy(view):
+    """vto_rotate_about (the order a device generator uses) reproduces
+    numpy.dot(p - c, R.T) + c of grid.generate_projection_grid bit for bit on
+    this host; the unrotated grid is linspace/meshgrid, pinned separately."""
+    v = dict(ROTATED_VIEWS[view])
+    s_rot, e_rot = grid.generate_projection_grid(**v)
+    base = dict(v)
+    c = base.pop("center")
+    s0, e0 = grid.generate_projection_grid(**base)  # same grid, no rotation
+    if "proj" in v:
+        yaw, pitch, roll, _ = grid._plane_angles(v["proj"])
+    else:
+        yaw, pitch, roll = v.get("yaw", 0.0), v.get("pitch", 0.0), v.get("roll", 0.0)
+    R = grid.rotation_matrix(yaw, pitch, roll)
+    np.testing.assert_array_equal(port.rotate_about(s0, R, c), s_rot.reshape(-1, 3))
+    np.testing.assert_array_equal(port.rotate_about(e0, R, c), e_rot.reshape(-1, 3))
+    # a plain sum in another order differs somewhere: the pin has teeth
+    a = s0.reshape(-1, 3) - np.asarray(c, float)
+    plain = (a[:, :1] * R[:, 0] + a[:, 1:2] * R[:, 1]) + a[:, 2:] * R[:, 2] + np.asarray(c, float)
+    if view == 0:
+        assert not np.array_equal(plain, s_rot.reshape(-1, 3))
+
+
 @needs_ref
 def test_pins_hold_unmutated():
     assert _pins_hold()

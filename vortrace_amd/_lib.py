@@ -25,6 +25,13 @@ class Tuning(C.Structure):
     _fields_ = [("walk_batch", C.c_int), ("leaf_batch", C.c_int), ("ray_parts", C.c_int)]
 
 
+class View(C.Structure):
+    _fields_ = [("extent", C.c_double * 4), ("bounds", C.c_double * 2),
+                ("center", C.c_double * 3), ("rot", C.c_double * 9),
+                ("nx", C.c_int64), ("ny", C.c_int64), ("rotate", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
 class CallStats(C.Structure):
     _fields_ = [("segments", C.c_int64), ("queries", C.c_int64),
                 ("max_walk_depth", C.c_int64), ("max_knn_depth", C.c_int64),
@@ -67,6 +74,7 @@ SIGNATURES = {
     "vt_slice": (I, [P, P, S, S, C.c_double, P, P]),
     "vt_adjoint": (I, [P, P, P, S, I, P, P, P, P]),
     "vt_project_counted": (I, [P, P, P, S, I, P, P, P]),
+    "vt_grid_rays": (I, [C.POINTER(View), S, P, P, P]),
     "vt_project_recorded": (I, [P, P, P, S, I, P, P, P, S, C.POINTER(S), P]),
     "vt_adjoint_recorded": (I, [P, S, I, P, P, S, P, P, P]),
     "vt_set_tuning": (I, [C.POINTER(Tuning)]),

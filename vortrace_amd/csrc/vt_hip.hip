@@ -864,6 +864,69 @@ int check_mode(const vt_cloud *c, int mode) {
 
 }  // namespace
 
+namespace vt {
+
+// numpy.linspace(a, b, n)[i] (numpy/_core/function_base.py): step =
+// (b - a) / (n - 1), y = i * step + a (i / (n - 1) * (b - a) + a when the
+// step underflows to 0), the last element b; n == 1 gives a.
+__device__ __forceinline__ double np_linspace(double a, double b, int64_t n, int64_t i) {
+  if (n > 1 && i == n - 1) return b;
+  const double delta = b - a;
+  if (n <= 1) return (double)i * delta + a;
+  const double div = (double)(n - 1);
+  const double step = delta / div;
+  return step == 0.0 ? ((double)i / div) * delta + a : (double)i * step + a;
+}
+
+// Pixel centres of grid._generate_base_grid (vortrace/grid.py:35-44, 47-83):
+// linspace(lo + d/2, hi - d/2, n) with d = (hi - lo) / n.
+__device__ __forceinline__ double pixel_centre(double lo, double hi, int64_t n, int64_t i) {
+  const double d = (hi - lo) / (double)n;
+  return np_linspace(lo + 0.5 * d, hi - 0.5 * d, n, i);
+}
+
+// One ray of grid.generate_projection_grid (vortrace/grid.py:100-192) per
+// thread: ray i * ny + j of its view starts at (x_i, y_j, bounds[0]) and ends
+// at (x_i, y_j, bounds[1]); with a rotation, p -> R (p - c) + c where
+// numpy.dot(p - c, R.T) evaluates each row as fma(p2, R_k2, fma(p1, R_k1,
+// p0 * R_k0)) (OpenBLAS' x86-64 kernels behind numpy).
+__global__ void k_grid_rays(const vt_view *views, const int64_t *offsets, int nviews,
+                            int64_t nrays, double *start, double *end) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrays) return;
+  int lo = 0, hi = nviews - 1;  // the view holding ray r: offsets[v] <= r < offsets[v + 1]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (gld(offsets + mid) <= r) lo = mid; else hi = mid - 1;
+  }
+  const vt_view *v = views + lo;
+  const int64_t local = r - gld(offsets + lo);
+  const int64_t ny = v->ny;
+  const int64_t i = local / ny, j = local - i * ny;
+  const double x = pixel_centre(v->extent[0], v->extent[1], v->nx, i);
+  const double y = pixel_centre(v->extent[2], v->extent[3], ny, j);
+  double p[2][3] = {{x, y, v->bounds[0]}, {x, y, v->bounds[1]}};
+  if (v->rotate) {
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const double a0 = p[e][0] - v->center[0], a1 = p[e][1] - v->center[1],
+                   a2 = p[e][2] - v->center[2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double *R = v->rot + 3 * k;
+        p[e][k] = __fma_rn(a2, R[2], __fma_rn(a1, R[1], a0 * R[0])) + v->center[k];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    gst(start + 3 * r + k, p[0][k]);
+    gst(end + 3 * r + k, p[1][k]);
+  }
+}
+
+}  // namespace vt
+
 namespace {
 
 // Device buffers of one adjoint call.
@@ -1616,6 +1679,39 @@ int vt_adjoint_recorded(const vt_cloud *cc, size_t nrays, int mode, const int64_
     HIP_OK(hipEventElapsedTime(&ms1, tm.e0, tm.e1));
     g_stats.kernel_ms = ms1 + ms2;
     g_stats.segments = nseg;
+    return VT_OK;
+  });
+}
+
+int vt_grid_rays(const vt_view *views, size_t nviews, double *d_start, double *d_end,
+                 void *stream) {
+  return guarded([&]() -> int {
+    if (nviews && !views) return fail(VT_EINVAL, "null argument");
+    std::vector<int64_t> off(nviews + 1, 0);
+    for (size_t v = 0; v < nviews; ++v) {
+      const vt_view &w = views[v];
+      if (w.nx < 1 || w.ny < 1) return fail(VT_EINVAL, "every view needs nx, ny >= 1");
+      if (w.nx > (int64_t)1 << 31 || w.ny > (int64_t)1 << 31 ||
+          w.nx * w.ny > ((int64_t)1 << 40))
+        return fail(VT_EINVAL, "view too large");
+      off[v + 1] = off[v] + w.nx * w.ny;
+    }
+    const int64_t n = off[nviews];
+    if (n == 0) return VT_OK;
+    if (!d_start || !d_end) return fail(VT_EINVAL, "null argument");
+    if (nviews > (size_t)INT32_MAX) return fail(VT_EINVAL, "too many views");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    DevBufs B;
+    vt_view *dv = B.get<vt_view>(nviews);
+    int64_t *doff = B.get<int64_t>(nviews + 1);
+    HIP_OK(hipMemcpyAsync(dv, views, nviews * sizeof(vt_view), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(doff, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                          st));
+    const int threads = 256;
+    k_grid_rays<<<(unsigned)((n + threads - 1) / threads), threads, 0, st>>>(
+        dv, doff, (int)nviews, n, d_start, d_end);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));  // the view table is freed on return
     return VT_OK;
   });
 }

@@ -97,3 +97,63 @@ def generate_projection_grid(extent: ArrayLike, nres, bounds: ArrayLike,
         start += center
         end += center
     return start, end
+
+
+def view_params(extent: ArrayLike, nres, bounds: ArrayLike,
+                center: ArrayLike | None = None, *, proj: str | None = None,
+                yaw: float = 0., pitch: float = 0., roll: float = 0.):
+    """The parameters of generate_projection_grid(...) as one C view record
+    (vt_view) for the device generator: extent, resolution, the bounds after
+    the handedness flip, and the rotation matrix (computed here, by numpy,
+    exactly as generate_projection_grid computes it)."""
+    from . import _lib
+    ext = np.array(extent)
+    if ext.ravel().size == 2:
+        ext = np.array([ext, ext])
+    elif ext.ravel().size != 4:
+        raise ValueError("extent must be either (2,2) or (2,) array")
+    if isinstance(nres, (int, np.integer)):
+        nres = (int(nres), int(nres))
+    if proj is not None:
+        yaw, pitch, roll, hand = _plane_angles(proj)
+    else:
+        hand = np.sign(bounds[1] - bounds[0])
+    if hand != np.sign(bounds[1] - bounds[0]):
+        bounds = np.flip(bounds)
+    v = _lib.View()
+    for k, val in enumerate((ext[0][0], ext[0][1], ext[1][0], ext[1][1])):
+        v.extent[k] = float(val)
+    v.bounds[0], v.bounds[1] = float(bounds[0]), float(bounds[1])
+    v.nx, v.ny = int(nres[0]), int(nres[1])
+    if center is not None:
+        c = np.zeros(3) + np.asarray(center)  # float64, as start -= center promotes
+        rot = rotation_matrix(yaw, pitch, roll)
+        for k in range(3):
+            v.center[k] = float(c[k])
+        for k in range(9):
+            v.rot[k] = float(rot.flat[k])
+        v.rotate = 1
+    return v
+
+
+def generate_projection_grids_device(views, device=None):
+    """generate_projection_grid for many views at once on the device
+    (vt_grid_rays).  `views` is a list of dicts of generate_projection_grid's
+    arguments.  Returns (start, end, sizes): (R, 3) float64 device tensors
+    holding every view's rays in turn (view v's ray i * ny + j is its pixel
+    (i, j)) and the rays per view."""
+    import ctypes as C
+    import torch
+    from . import _lib
+    dev = torch.device(device) if device is not None else torch.device("cuda")
+    recs = [view_params(**v) for v in views]
+    sizes = [int(r.nx * r.ny) for r in recs]
+    n = sum(sizes)
+    start = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    end = torch.empty((n, 3), dtype=torch.float64, device=dev)
+    arr = (_lib.View * max(1, len(recs)))(*recs)
+    with torch.cuda.device(dev):
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        _lib.check(_lib.lib().vt_grid_rays(arr, len(recs), C.c_void_p(start.data_ptr()),
+                                           C.c_void_p(end.data_ptr()), C.c_void_p(stream)))
+    return start, end, sizes

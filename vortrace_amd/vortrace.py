@@ -210,6 +210,28 @@ class ProjectionCloud:
         out = Projection(torch.cat(starts), torch.cat(ends)).project_tensor(self._cloud, mode)
         return list(torch.split(out, sizes))
 
+    def batch_grid_projection(self, views, *, reduction: str = "integrate"):
+        """grid_projection for many views in ONE launch, with the rays built
+        on the device (grid.generate_projection_grids_device) from each
+        view's grid_projection arguments (a list of dicts: extent, nres,
+        bounds, center, proj, yaw, pitch, roll).  Returns one device tensor
+        per view shaped like grid_projection's result."""
+        mode = self._validate_reduction(reduction)
+        self._require_tree()
+        dev = self._cloud._device
+        start, end, sizes = gr.generate_projection_grids_device(views, dev)
+        if not sizes:
+            return []
+        out = Projection(start, end).project_tensor(self._cloud, mode)
+        nout = self._out_fields(mode)
+        res = []
+        for v, o in zip(views, torch.split(out, sizes)):
+            nres = v["nres"]
+            nx, ny = (nres, nres) if isinstance(nres, (int, np.integer)) else nres
+            res.append(o.reshape(int(nx), int(ny)) if nout == 1 else
+                       o.reshape(int(nx), int(ny), nout))
+        return res
+
     def batch_projection(self, views, *, reduction: str = "integrate"):
         """batch_projection_tensor with numpy results shaped like
         ``projection``'s: (n_i,) for one output field, else (n_i, n_out)."""
