@@ -82,13 +82,18 @@ def make_workload(name):
         pos, fields = synthetic.clustered_mesh(w["n"], seed=w.get("seed", 2))
     log(f"[bench] mesh {name}: {len(pos)} points in {time.perf_counter() - t0:.1f}s "
         f"(sha {synthetic.digest(pos, fields)})")
+    params = None
     if "views" in w:
-        views = synthetic.camera_rays(w["cam_seed"], w["views"], sizes=w["sizes"],
-                                      extent_range=w["extent"])
+        params = synthetic.camera_views(w["cam_seed"], w["views"], sizes=w["sizes"],
+                                        extent_range=w["extent"])
+        views = []
+        for v in params:
+            s, e = grid.generate_projection_grid(**v)
+            views.append((s.reshape(-1, 3), e.reshape(-1, 3)))
     else:
         s, e = grid.generate_projection_grid([0.0, 100.0], w["npix"], [0.0, 100.0], None)
         views = [(s.reshape(-1, 3), e.reshape(-1, 3))]
-    return pos, fields, views
+    return pos, fields, views, params
 
 
 def algorithmic_bytes(R, C, n_points, n_f, n_out):
@@ -335,7 +340,7 @@ def main():
 
     w = WORKLOADS[args.workload]
     kind, mode = w.get("kind", "image"), w["mode"]
-    pos, fields, views = make_workload(args.workload)
+    pos, fields, views, params = make_workload(args.workload)
     if mode == "volume":
         # kappa: median ray opacity 0.5 on a 64^2 z-probe of the box
         pc_probe = vt.ProjectionCloud(pos, fields, boundbox=[0, 100, 0, 100, 0, 100],
@@ -357,12 +362,24 @@ def main():
     info = pc._cloud.tree_info()
     dev = torch.device("cuda", 0)
     t0 = time.perf_counter()
-    d_views = [(torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)) for s, e in views]
-    if len(d_views) == 1:
-        d_start, d_end = d_views[0]
+    ray_gen = None
+    if params is not None:
+        # camera rays built on the device from the views' parameters
+        d_start, d_end, sizes = vt.grid.generate_projection_grids_device(params, dev)
+        torch.cuda.synchronize()
+        t_gen = time.perf_counter() - t0
+        same = all(torch.equal(d_start[o:o + n].cpu(), torch.from_numpy(s)) and
+                   torch.equal(d_end[o:o + n].cpu(), torch.from_numpy(e))
+                   for (s, e), o, n in zip(views, np.cumsum([0] + sizes[:-1]), sizes))
+        d_views = list(zip(torch.split(d_start, sizes), torch.split(d_end, sizes)))
+        ray_gen = {"device_seconds": t_gen, "bit_identical_to_numpy": bool(same)}
+        log(f"[bench] {len(sizes)} views' rays built on the device in {t_gen:.3f}s; "
+            f"identical to numpy's: {same}")
+        t0 = time.perf_counter()
     else:
-        d_start = torch.cat([v[0] for v in d_views])
-        d_end = torch.cat([v[1] for v in d_views])
+        d_views = [(torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev))
+                   for s, e in views]
+        d_start, d_end = d_views[0]
     torch.cuda.synchronize()
     t_h2d = time.perf_counter() - t0
     start = np.concatenate([v[0] for v in views]) if len(views) > 1 else views[0][0]
@@ -456,6 +473,7 @@ def main():
         "tuning": tuning,
         "setup_seconds": {"tree_build_host": info["build_seconds"],
                           "cloud_and_upload": t_build, "rays_h2d": t_h2d},
+        "ray_generation": ray_gen,
         "cpu_baseline": None,
     }
     if not args.no_cpu:

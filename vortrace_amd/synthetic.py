@@ -140,27 +140,36 @@ def kappa_for_median_opacity(rho, cell_ids, ds, offsets, target=0.5):
     return np.sqrt(lo * hi)
 
 
-def camera_rays(rng_seed: int, n_views: int, box: float = 100.0,
-                sizes=(64, 128, 256, 512, 1024), extent_range=(10.0, 100.0)):
-    """Per-view (start, end) rays through the box centre with isotropic
-    directions, as the reference's grid recipe generates them
-    (grid.generate_projection_grid with yaw = atan2(dy, dx),
-    pitch = arccos(dz), roll = 0, rotated about the centre)."""
-    from . import grid
-
+def camera_views(rng_seed: int, n_views: int, box: float = 100.0,
+                 sizes=(64, 128, 256, 512, 1024), extent_range=(10.0, 100.0)):
+    """Per-view grid.generate_projection_grid arguments: views through the
+    box centre with isotropic directions d (yaw = atan2(d_y, d_x),
+    pitch = arccos(d_z), roll = 0, so that R z = d), extents uniform in
+    extent_range and image sizes drawn from `sizes`."""
     rng = np.random.default_rng(rng_seed)
     dirs = _isotropic(rng, n_views)
     ext = extent_range[0] + (extent_range[1] - extent_range[0]) * _u(rng, n_views)
     pick = np.minimum((len(sizes) * _u(rng, n_views)).astype(int), len(sizes) - 1)
-    c = np.array([box / 2] * 3)
+    c = [box / 2] * 3
     views = []
     for v in range(n_views):
         d = dirs[v]
-        yaw = np.arctan2(d[1], d[0])
-        pitch = np.arccos(np.clip(d[2], -1.0, 1.0))
         e = ext[v]
-        s, t = grid.generate_projection_grid([box / 2 - e / 2, box / 2 + e / 2],
-                                             int(sizes[pick[v]]), [0.0, box], c,
-                                             yaw=yaw, pitch=pitch, roll=0.0)
+        views.append(dict(extent=[box / 2 - e / 2, box / 2 + e / 2],
+                          nres=int(sizes[pick[v]]), bounds=[0.0, box], center=c,
+                          yaw=float(np.arctan2(d[1], d[0])),
+                          pitch=float(np.arccos(np.clip(d[2], -1.0, 1.0))), roll=0.0))
+    return views
+
+
+def camera_rays(rng_seed: int, n_views: int, box: float = 100.0,
+                sizes=(64, 128, 256, 512, 1024), extent_range=(10.0, 100.0)):
+    """(start, end) host rays of camera_views(...), each (n_i, 3), as the
+    reference's grid recipe generates them."""
+    from . import grid
+
+    views = []
+    for v in camera_views(rng_seed, n_views, box, sizes, extent_range):
+        s, t = grid.generate_projection_grid(**v)
         views.append((s.reshape(-1, 3), t.reshape(-1, 3)))
     return views
