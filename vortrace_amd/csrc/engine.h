@@ -60,13 +60,16 @@ struct Search {
   int m;
   int img;  // -1: the plain search; else the image (ix+1)*9 + (iy+1)*3 + (iz+1)
 
-  // computeInitialDistances against root_bbox, then the first descent.
+  // computeInitialDistances against root_bbox, then the first descent.  With
+  // a bound, only points closer than it are looked for (a result set
+  // pre-filled with `bound` and no ids).
   __device__ __forceinline__ void begin_search(const DevCloud &C, KStack &st, double x,
-                                               double y, double z) {
+                                               double y, double z,
+                                               double bound = DBL_MAX) {
     q[0] = x;
     q[1] = y;
     q[2] = z;
-    rs.init();
+    rs.init(bound);
     mind = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -305,7 +308,7 @@ struct Search {
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-          if (i < k) {
+          if (i < k && rs.id[i] != kNone) {
             bool found = false;
 #pragma unroll
             for (int j = 0; j < 6; ++j) {
@@ -355,7 +358,11 @@ struct Search {
                                 qb[2] + iz * C.box[2]};
           const double wk = sel3(md[0], md[1], md[2], (uint32_t)(k - 1));
           if (min_dist_sq_to_box(C, sh) > wk + kTol) continue;
-          begin_search(C, st, sh[0], sh[1], sh[2]);
+          // An image's neighbours at r^2 >= wk cannot change the merged k
+          // (a duplicate keeps its smaller r^2; a newcomer sorts after the
+          // k entries <= wk and is truncated), so the image search looks
+          // only below wk: the same merge, with most of the tree pruned.
+          begin_search(C, st, sh[0], sh[1], sh[2], wk);
           return false;
         }
 #pragma unroll

@@ -172,13 +172,13 @@ struct RegRes3 {
   int k;
   double w;
 
-  __device__ __forceinline__ void init() {
+  __device__ __forceinline__ void init(double bound = DBL_MAX) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      d[j] = DBL_MAX;
+      d[j] = bound;
       id[j] = kNone;
     }
-    w = DBL_MAX;
+    w = bound;
   }
   __device__ __forceinline__ double worst() const { return w; }
   // addPoint for dist < worst() (a smaller distance always lands in the first
