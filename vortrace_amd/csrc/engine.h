@@ -93,13 +93,13 @@ struct Search {
   }
 
   __device__ __forceinline__ void start(const DevCloud &C, KStack &st, const double (&x)[3],
-                                        int k) {
+                                        int k, double bound = DBL_MAX) {
     qb[0] = x[0];
     qb[1] = x[1];
     qb[2] = x[2];
     rs.k = k;
     img = -1;
-    begin_search(C, st, x[0], x[1], x[2]);
+    begin_search(C, st, x[0], x[1], x[2], bound);
   }
 
   // Component f of q / dd.  Values are selected, never addresses: a select
@@ -427,13 +427,14 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
       bool query = false;
       double q[3];
       int k = 1;
+      double bound = DBL_MAX;
       if (ph == kPhWalk) {
         uint32_t ids[3];
         double r2[3];
         if (Q.finish(C, st, ids, r2)) {
           ++cnt.nq;
           rc = W.template consume<kExpand>(C, S, wbase, (int)threadIdx.x, gtid, ids, r2, sk, cnt.maxw, cnt.nslow, e);
-          if (!rc && W.state < kStFinal) rc = W.prepare(C, S, wbase, end + 3 * ray, q, k, e);
+          if (!rc && W.state < kStFinal) rc = W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e);
           if (!rc && W.state == kStFinal) {
             sk.seg(W.open_cell, W.entry_s, W.sc);
             W.state = kStDone;
@@ -475,12 +476,12 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
           if (ray_list) ray = (int64_t)gld(ray_list + ray);
           W.begin(start + 3 * ray, end + 3 * ray);
           io.begin(ray, sk);
-          W.prepare(C, S, wbase, end + 3 * ray, q, k, e);  // kStStart: the ray's start
+          W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e);  // kStStart: the ray's start
           query = true;
         }
       }
       if (query) {
-        Q.start(C, st, q, k);
+        Q.start(C, st, q, k, bound);
         ph = kPhSearch;
       }
     }

@@ -74,15 +74,20 @@ __device__ __forceinline__ double split_analytic(const RayGeom &r, const double 
 // The analytic part of findSplitPointDistance(cloud, id1, id2, s_lo, s_hi)
 // (src/ray.cpp:64-94): minimum-image generators in periodic mode, then the
 // bisector.  Returns false when the reference falls back to bisection.
+// p1 / p2 receive the two generators as used (minimum images).
 template <bool kPer>
 __device__ __forceinline__ bool split_point(const DevCloud &C, const RayGeom &r, uint32_t id1,
                                             uint32_t id2, double s_lo, double s_hi,
-                                            double &out) {
+                                            double &out, double (&p1)[3], double (&p2)[3]) {
   const double2 *g1 = reinterpret_cast<const double2 *>(C.gen + id1);
   const double2 *g2 = reinterpret_cast<const double2 *>(C.gen + id2);
   const double2 a1 = gld(g1), b1 = gld(g1 + 1), a2 = gld(g2), b2 = gld(g2 + 1);
-  double p1[3] = {a1.x, a1.y, b1.x};
-  double p2[3] = {a2.x, a2.y, b2.x};
+  p1[0] = a1.x;
+  p1[1] = a1.y;
+  p1[2] = b1.x;
+  p2[0] = a2.x;
+  p2[1] = a2.y;
+  p2[2] = b2.x;
   if constexpr (kPer) {
     const double s_ref = 0.5 * (s_lo + s_hi);
 #pragma unroll
@@ -292,9 +297,22 @@ struct Walker {
   }
 
   // Advances to the next query: on success with state < kStFinal, q and k are
-  // the query; with kStFinal the ray has no query left.
+  // the query; with kStFinal the ray has no query left.  `bound`: the search
+  // need only find neighbours closer than it (DBL_MAX: all of them).
+  //
+  // The classification after an analytic split point reads its 3-NN result
+  // only through the entries within kTol of the nearest (ids and r^2) and
+  // through the tests r2[i] - r2[0] >= kTol, r2[0] + kTol < r2[1] for the
+  // rest.  The nearest is at most as far as the nearer of the two cells the
+  // split point lies between, m; so with bound B = m (1 + 1e-12) + 2 kTol
+  // every entry within kTol of the nearest is found exactly, and an entry at
+  // or beyond B (left as the placeholder (none, B)) passes both tests as
+  // the true one does.  When 2 kTol is below the resolution of m, no bound.
   __device__ __forceinline__ int prepare(const DevCloud &C, const Scratch &S, size_t wbase, const double *end,
-                         double (&q)[3], int &k, WalkErr &e) {
+                         double (&q)[3], int &k, double &bound, WalkErr &e) {
+    bound = DBL_MAX;
+    double p1[3], p2[3];
+    bool have_pair = false;
     if (state == kStSplit) {
       for (;;) {
         if (++iteration > kMaxIter) return kErrMaxIter;
@@ -308,9 +326,10 @@ struct Walker {
           continue;
         }
         double sv;
-        if (split_point<kPer>(C, r, ct, nt, sc, sn, sv)) {
+        if (split_point<kPer>(C, r, ct, nt, sc, sn, sv, p1, p2)) {
           const int rc = accept_split(sv, e);
           if (rc) return rc;
+          have_pair = true;
         } else {
           lo = sc;
           hi = sn;
@@ -343,6 +362,18 @@ struct Walker {
 #pragma unroll
         for (int j = 0; j < 3; ++j) q[j] = pos[j];
         k = 3;
+        if (have_pair) {
+          double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double e1 = pos[j] - p1[j], e2 = pos[j] - p2[j];
+            d1 = d1 + e1 * e1;
+            d2 = d2 + e2 * e2;
+          }
+          const double m = (d1 < d2 ? d1 : d2) * (1.0 + 1e-12);
+          const double b = m + 2.0 * kTol;
+          if (b - m >= 1.5 * kTol) bound = b;
+        }
       } else if (state == kStAlongC || state == kStAlongN) {
         const double sign = state == kStAlongC ? -1.0 : 1.0;
 #pragma unroll
