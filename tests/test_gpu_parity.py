@@ -263,3 +263,21 @@ def test_knn_tiny_periodic_duplicates():
         oi, orr = oc.query(q, k)
         np.testing.assert_array_equal(ids.cpu().numpy(), oi)
         np.testing.assert_array_equal(r2.cpu().numpy(), orr)
+
+
+@pytest.mark.parametrize("scale", [1e-3, 1e5, 1e7])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_bounded_searches_across_scales(scale, periodic):
+    """The split-point classification searches are bounded by the nearer
+    cell's r^2 plus 2 kTol (kTol = 1e-9, absolute): at box 1e-3 the bound is
+    mostly the 2 kTol; at 1e5 2 kTol is near the resolution of r^2; at 1e7 it
+    is below it and the searches run unbounded.  Bits must match everywhere."""
+    _say(f"bounded searches scale={scale} periodic={periodic}")
+    pos, rho = synthetic.uniform_mesh(3000, seed=15)
+    pos = pos * (scale / 100.0)
+    bb = [0, scale, 0, scale, 0, scale]
+    pc = vt.ProjectionCloud(pos, rho, boundbox=bb, periodic=periodic)
+    oc = port.OracleCloud(pos, rho, bb, periodic=periodic)
+    s, e = grid.generate_projection_grid([0.0, scale], 12, [0.0, scale], None)
+    off = _cmp_segments(pc, oc, s.reshape(-1, 3), e.reshape(-1, 3))
+    assert off[-1] > 2000
