@@ -47,7 +47,7 @@ struct Search {
   RegRes3 rs;
   uint32_t cur;  // inner node index, or leaf reference
   int mode;
-  int depth;            // >= 0 on the first descent (LDS ring depth), else -1
+  int depth;            // >= 0 on the first descent (entries in the LDS ring), else -1
   double lost_min;      // smallest box distance that fell out of the ring
   uint32_t first_leaf;  // first descent's leaf; a re-descent stops there
   double pv;            // popped entry: box distance of the far child
@@ -124,19 +124,22 @@ struct Search {
     const double m2 = mind + cut - ddd(f);
     const uint32_t tag = far_tag(low_first ? nd.c2 : nd.c1, f);
     if (depth >= 0) {
-      // First descent (worst distance still infinite): every far child goes
-      // into the LDS ring at slot depth mod kLdsDepth.
-      const int sl = st.ring_slot(depth);
-      if (depth >= kLdsDepth) {
-        uint32_t ot;
-        double old;
-        float op;
-        unpack_entry(lds_entry(sl), ot, old, op);
-        lost_min = old < lost_min ? old : lost_min;
+      // First descent (no point found yet, worst = the search's bound): every
+      // far child that can still be visited goes into the LDS ring, the
+      // depth-th at slot depth mod kLdsDepth; one beyond the bound never can.
+      if (m2 <= rs.worst()) {
+        const int sl = st.ring_slot(depth);
+        if (depth >= kLdsDepth) {
+          uint32_t ot;
+          double old;
+          float op;
+          unpack_entry(lds_entry(sl), ot, old, op);
+          lost_min = old < lost_min ? old : lost_min;
+        }
+        set_lds_entry(sl, pack_entry(tag, m2, ptop));
+        ptop = fminf(ptop, bound_below(m2));
+        ++depth;
       }
-      set_lds_entry(sl, pack_entry(tag, m2, ptop));
-      ptop = fminf(ptop, bound_below(m2));
-      ++depth;
     } else if (m2 <= rs.worst()) {
       st.push(tag, m2, ptop);
       ptop = fminf(ptop, bound_below(m2));
