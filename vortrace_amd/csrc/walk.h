@@ -165,13 +165,17 @@ struct WalkErr {
 };
 
 // One ray's walk.  Per-lane scratch: the pending stack below its top lives in
-// S.walk_tag / S.walk_s at [depth * S.lanes + wbase].
+// S.walk_tag / S.walk_s at [depth * S.lanes + wbase]; the entry right below
+// the top is also kept in registers, so taking it off the stack does not wait
+// for memory (its replacement is read then, and needed one pop later).
 template <bool kPer>
 struct Walker {
   RayGeom r;
   uint32_t ct, nt;  // current point, and the top of the pending stack ("next")
   double sc, sn;
-  int wsp;  // pending entries below the top, in device memory
+  int wsp;  // pending entries below the top (entries 0 .. wsp-2 in device memory)
+  uint32_t nt2;  // entry wsp-1, when wsp > 0
+  double sn2;
   uint32_t open_cell;
   double entry_s;
   int iteration;
@@ -190,6 +194,8 @@ struct Walker {
     sc = 0.0;
     sn = r.len;
     wsp = 0;
+    nt2 = 0;
+    sn2 = 0.0;
     open_cell = 0;
     entry_s = 0.0;
     iteration = 0;
@@ -203,8 +209,12 @@ struct Walker {
   __device__ __forceinline__ bool push_top(const Scratch &S, size_t wbase, int &max_depth,
                                            uint32_t cell, double sv) {
     if (wsp >= S.wcap) return false;
-    gst(S.walk_tag + (size_t)wsp * S.lanes + wbase, nt);
-    gst(S.walk_s + (size_t)wsp * S.lanes + wbase, sn);
+    if (wsp > 0) {
+      gst(S.walk_tag + (size_t)(wsp - 1) * S.lanes + wbase, nt2);
+      gst(S.walk_s + (size_t)(wsp - 1) * S.lanes + wbase, sn2);
+    }
+    nt2 = nt;
+    sn2 = sn;
     ++wsp;
     max_depth = wsp > max_depth ? wsp : max_depth;
     nt = cell;
@@ -217,8 +227,12 @@ struct Walker {
     sc = sn;
     if (wsp == 0) return false;
     --wsp;
-    nt = gld(S.walk_tag + (size_t)wsp * S.lanes + wbase);
-    sn = gld(S.walk_s + (size_t)wsp * S.lanes + wbase);
+    nt = nt2;
+    sn = sn2;
+    if (wsp > 0) {
+      nt2 = gld(S.walk_tag + (size_t)(wsp - 1) * S.lanes + wbase);
+      sn2 = gld(S.walk_s + (size_t)(wsp - 1) * S.lanes + wbase);
+    }
     return true;
   }
   // The split point was found (analytically or by bisection): bounds check,
