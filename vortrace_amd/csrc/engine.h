@@ -354,13 +354,36 @@ struct Search {
         ++img;
       }
       if (img >= 0) {
-        for (; img < 27; ++img) {
-          if (img == 13) continue;
+        // Images img..26 in loop order, skipping at once those with a shifted
+        // coordinate whose own term already exceeds wk + kTol (their
+        // minDistSqToBox does too); the rest get the reference's exact test,
+        // minDistSqToBox(shifted) > wk + kTol, as ib.d2, which adds the same
+        // per-dimension terms in the same order.
+        ImageBox ib;
+        ib.init(C, qb);
+        uint32_t cand;
+        {
+          const double w = sel3(md[0], md[1], md[2], (uint32_t)(k - 1)) + kTol;
+          uint32_t ms[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            ms[d] = 2u | (ib.lo[d] <= w ? 1u : 0u) | (ib.hi[d] <= w ? 4u : 0u);
+          cand = 0u;
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+              if (((ms[0] >> a) & (ms[1] >> b) & 1u) != 0u) cand |= ms[2] << (a * 9 + b * 3);
+          cand &= ~(1u << 13) & ~((1u << img) - 1u);
+        }
+        while (cand) {
+          img = __ffs(cand) - 1;
+          cand &= cand - 1u;
           const int ix = img / 9 - 1, iy = (img / 3) % 3 - 1, iz = img % 3 - 1;
+          const double wk = sel3(md[0], md[1], md[2], (uint32_t)(k - 1));
+          if (ib.d2(ix, iy, iz) > wk + kTol) continue;
           const double sh[3] = {qb[0] + ix * C.box[0], qb[1] + iy * C.box[1],
                                 qb[2] + iz * C.box[2]};
-          const double wk = sel3(md[0], md[1], md[2], (uint32_t)(k - 1));
-          if (min_dist_sq_to_box(C, sh) > wk + kTol) continue;
           // An image's neighbours at r^2 >= wk cannot change the merged k
           // (a duplicate keeps its smaller r^2; a newcomer sorts after the
           // k entries <= wk and is truncated), so the image search looks
@@ -368,6 +391,7 @@ struct Search {
           begin_search(C, st, sh[0], sh[1], sh[2], wk);
           return false;
         }
+        img = 27;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           ids[j] = j < k ? mi[j] : kNone;
