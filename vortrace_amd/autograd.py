@@ -83,9 +83,12 @@ class ProjectFields(torch.autograd.Function):
         # Statistics are per thread, and autograd may run this on its own.
         pc._adjoint_stats = _lib.last_call_stats()
         ctx.records = None
-        full = torch.zeros((fields.shape[0], gf.shape[1]), dtype=torch.float64,
-                           device=gf.device)
-        full[pc._orig_ids_tensor(gf.device)] = gf
+        if pc._keeps_all_in_order():
+            full = gf
+        else:
+            full = torch.zeros((fields.shape[0], gf.shape[1]), dtype=torch.float64,
+                               device=gf.device)
+            full[pc._orig_ids_tensor(gf.device)] = gf
         return full.reshape(fields.shape).to(fields.dtype), None, None, None, None
 
 

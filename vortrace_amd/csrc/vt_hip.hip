@@ -576,6 +576,9 @@ struct vt_cloud {
   size_t defer_cap = 0;
   int kspill = 0;
   DevCloud dc{};
+  // Work buffers of the adjoint, recording and field-update calls, kept
+  // between calls (slot i of a call's i-th request, grown when too small).
+  std::vector<std::pair<void *, size_t>> arena;
 
   void free_device() {
     for (void *p : {(void *)d_nodes, (void *)d_recs, (void *)d_gen, (void *)d_fields, (void *)d_slot_id,
@@ -583,6 +586,9 @@ struct vt_cloud {
                     (void *)scratch.walk_tag, (void *)scratch.walk_s,
                     (void *)scratch.big_id, (void *)scratch.big_d, (void *)d_defer})
       if (p) (void)hipFree(p);
+    for (auto &a : arena)
+      if (a.first) (void)hipFree(a.first);
+    arena.clear();
     d_defer = nullptr;
     defer_cap = 0;
     d_nodes = nullptr;
@@ -929,14 +935,34 @@ __global__ void k_grid_rays(const vt_view *views, const int64_t *offsets, int nv
 
 namespace {
 
-// Device buffers of one adjoint call.
+// Device buffers of one call.  With a cloud, the i-th request of a call is
+// served from the cloud's arena slot i (grown by a quarter more than asked
+// when too small, so repeated calls allocate nothing); without one, each
+// request is allocated and freed with the call.
 struct DevBufs {
+  vt_cloud *c = nullptr;
+  size_t next = 0;
   std::vector<void *> ptrs;
+  explicit DevBufs(vt_cloud *cloud = nullptr) : c(cloud) {}
   template <class T>
   T *get(size_t count) {
-    T *p = dev_alloc<T>(std::max<size_t>(1, count));
-    ptrs.push_back(p);
-    return p;
+    const size_t bytes = std::max<size_t>(1, count) * sizeof(T);
+    if (!c) {
+      T *p = dev_alloc<T>(std::max<size_t>(1, count));
+      ptrs.push_back(p);
+      return p;
+    }
+    if (next == c->arena.size()) c->arena.push_back({nullptr, 0});
+    auto &slot = c->arena[next++];
+    if (slot.second < bytes) {
+      if (slot.first) (void)hipFree(slot.first);
+      slot.first = nullptr;
+      slot.second = 0;
+      const size_t grown = bytes + bytes / 4;
+      slot.first = dev_alloc<char>(grown);
+      slot.second = grown;
+    }
+    return static_cast<T *>(slot.first);
   }
   ~DevBufs() {
     for (void *p : ptrs) (void)hipFree(p);
@@ -1381,7 +1407,7 @@ int vt_project_recorded(const vt_cloud *cc, const double *d_start, const double 
     if (nrays == 0) return VT_OK;
     const int nout = mode == VT_VOLUME ? 3 : (int)c->nf;
     const int64_t n = (int64_t)nrays;
-    DevBufs B;
+    DevBufs B(c);
     unsigned long long *head = B.get<unsigned long long>(1);
     int *overflow = B.get<int>(1);
     HIP_OK(hipMemsetAsync(head, 0, sizeof *head, st));
@@ -1545,7 +1571,8 @@ int vt_cloud_set_fields(vt_cloud *c, const double *d_fields, void *stream) {
     if (c->n == 0) return VT_OK;
     if (!d_fields) return fail(VT_EINVAL, "null argument");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    int *flag = dev_alloc<int>(1);
+    DevBufs B(c);
+    int *flag = B.get<int>(1);
     HIP_OK(hipMemsetAsync(flag, 0, sizeof(int), st));
     const int threads = 256;
     const int blocks = (int)((c->n + threads - 1) / threads);
@@ -1555,7 +1582,6 @@ int vt_cloud_set_fields(vt_cloud *c, const double *d_fields, void *stream) {
     int invalid = 0;
     HIP_OK(hipMemcpyAsync(&invalid, flag, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
-    (void)hipFree(flag);
     c->valid_rgba = (c->nf == 4 && !invalid) ? 1 : 0;
     return VT_OK;
   });
@@ -1577,7 +1603,7 @@ int vt_adjoint(const vt_cloud *cc, const double *d_start, const double *d_end, s
     HIP_OK(hipMemsetAsync(d_grad_fields, 0, c->n * nf * sizeof(double), st));
     if (nrays == 0) return VT_OK;
     const int64_t n = (int64_t)nrays;
-    DevBufs B;
+    DevBufs B(c);
     // 1. segments per ray, and offsets
     int64_t *counts = B.get<int64_t>(nrays);
     int64_t *offsets = B.get<int64_t>(nrays + 1);
@@ -1655,7 +1681,7 @@ int vt_adjoint_recorded(const vt_cloud *cc, size_t nrays, int mode, const int64_
     g_stats = vt_call_stats{};
     if (nrays == 0) return VT_OK;
     const int64_t n = (int64_t)nrays;
-    DevBufs B;
+    DevBufs B(c);
     int64_t *offsets = B.get<int64_t>(nrays + 1);
     const int64_t nseg = scan_counts(d_counts, nrays, offsets, B, st);
     if (nseg > (int64_t)0xFFFFFFFFll)

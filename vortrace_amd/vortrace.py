@@ -266,7 +266,21 @@ class ProjectionCloud:
         f = f.to(device=dev, dtype=torch.float64).reshape(len(self.pos_orig), -1)
         if f.shape[1] != self._nfields:
             raise ValueError(f"fields must have {self._nfields} columns, got {f.shape[1]}")
-        self._cloud.set_fields_tensor(f[self._orig_ids_tensor(dev)].contiguous())
+        if self._keeps_all_in_order():
+            self._cloud.set_fields_tensor(f.contiguous())
+        else:
+            self._cloud.set_fields_tensor(f[self._orig_ids_tensor(dev)].contiguous())
+
+    def _keeps_all_in_order(self) -> bool:
+        """True when the filtered points are all input points in input order
+        (no padding filter took any out), so no reordering is needed."""
+        flag = getattr(self, "_ids_identity", None)
+        if flag is None:
+            ids = self.orig_ids
+            flag = len(ids) == len(self.pos_orig) and bool(
+                np.array_equal(ids, np.arange(len(ids), dtype=ids.dtype)))
+            self._ids_identity = flag
+        return flag
 
     def projection_grad(self, pos_start, pos_end, fields, *,
                         reduction: str = "integrate") -> torch.Tensor:
