@@ -13,6 +13,11 @@
  * RuntimeError).  vt_last_error() returns the message of the last failure on
  * the calling thread, with the reference's wording.
  *
+ * A cloud owns per-cloud device work memory (the kernels' per-lane stacks,
+ * the deferred-ray list, the adjoint's and field update's buffers, kept
+ * between calls): calls on one cloud must not overlap in time; different
+ * clouds may be used from different threads at once.
+ *
  * Cell ids crossing the boundary are indices into the filtered particle list,
  * as in the reference (PointCloud::orig_ids maps them to input rows).
  */
