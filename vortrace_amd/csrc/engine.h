@@ -176,7 +176,7 @@ struct Search {
   // Takes entries off the stack until one is visited (mode kTmEnter or
   // kTmLeaf) or nothing left can be (kTmDone).  The top kPopBatch entries are
   // read at once and resolved in registers, top first.
-  static constexpr int kPopBatch = 4;
+  static constexpr int kPopBatch = 2;
   __device__ __forceinline__ void pop(KStack &st) {
     const double worst = rs.worst();
     while (st.sp > 0) {
