@@ -103,6 +103,7 @@ struct Scratch {
   int walk_batch;      // ray engine: lanes of a wave that run their walk steps together
   int leaf_batch;      // ray engine: lanes of a wave that run their leaf steps together
   int parts;           // ray engine: ray partitions (1 or kParts)
+  int fetch_lanes;     // ray engine: lanes of each wave that take rays (1..64)
 };
 
 constexpr int kMaxQueryK = 128;

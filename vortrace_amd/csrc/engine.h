@@ -486,6 +486,7 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
       if (ph == kPhFetch) {
         // The lane's own partition first, then the others in turn.
         ray = -1;
+        if ((int)(threadIdx.x & 63) >= S.fetch_lanes) part_tries = nparts;
         while (part_tries < nparts) {
           const int64_t lo = part * psize;
           const int64_t idx = (int64_t)atomicAdd(&stats->part_counter[part], 1ULL);

@@ -823,10 +823,16 @@ int two_pass(vt_cloud *c, hipStream_t st, size_t nrays, Fast &&fast, Slow &&slow
   return finish_call(c, st, tm.e0, tm.e1, true);
 }
 
+// The ray engine's launches: a full persistent grid, and when there are
+// fewer rays than lanes, only the first fetch_lanes lanes of every wave take
+// rays, so the rays spread over all CUs instead of filling a few waves.
 template <class K, class... A>
 void launch(vt_cloud *c, K kernel, int64_t n, hipStream_t st, Timer &tm, A... args) {
-  const int grid = grid_for(kernel, n, c->device);
+  const int grid = grid_for(kernel, INT64_MAX / 2, c->device);
   ensure_scratch(c, (size_t)grid * kBlock);
+  const int64_t lanes = (int64_t)grid * kBlock;
+  c->scratch.fetch_lanes =
+      (int)std::min<int64_t>(64, std::max<int64_t>(1, (64 * n + lanes - 1) / lanes));
   HIP_OK(hipEventRecord(tm.e0, st));
   kernel<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, args...);
   HIP_OK(hipGetLastError());
