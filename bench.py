@@ -319,6 +319,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="multi-view workload: keep the current launch parameters")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ray-order", choices=("tiles", "rows"), default="tiles",
+                    help="tiles: the kernels take image rays in 8x8-pixel tiles when every "
+                         "view has the same row length; rows: in index order")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the forward+adjoint measurement")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
@@ -399,6 +402,17 @@ def main():
         f"{info['build_seconds']:.2f}s (cloud+upload {t_build:.2f}s); {len(views)} view(s), "
         f"{R} rays, H2D {t_h2d:.3f}s")
 
+    # every view's rays are a row-major image; with one row length for all,
+    # the kernels can take them in 8x8-pixel tiles (speed only)
+    if params is not None:
+        rows = {int(v["nres"]) if np.ndim(v["nres"]) == 0 else int(v["nres"][1])
+                for v in params}
+    else:
+        rows = {int(w["npix"])}
+    ray_row = rows.pop() if len(rows) == 1 and args.ray_order == "tiles" else 0
+    pc.set_ray_rows(ray_row)
+    log(f"[bench] ray order: {'8x8 tiles of %d-ray rows' % ray_row if ray_row else 'index'}")
+
     tuning = None
     if kind == "views" and not args.no_autotune:
         tuning = vt.tuning.autotune(pc, d_start, d_end, reduction=mode, stride=16,
@@ -464,7 +478,8 @@ def main():
         "config": {"workload": w["desc"], "step": "forward + adjoint" if kind == "fit"
                    else "forward", "n_points": len(pos), "views": len(views), "rays": R,
                    "crossings_per_step": C, "periodic": w["periodic"],
-                   "reduction": mode, "tree_depth": int(info["max_depth"])},
+                   "reduction": mode, "tree_depth": int(info["max_depth"]),
+                   "ray_order": f"8x8 tiles ({ray_row}-ray rows)" if ray_row else "index"},
         "rays_per_s": step["rays_per_s"],
         "kernel_ms": kernel_ms,
         "kernel_crossings_per_s": C / (kernel_ms * 1e-3),

@@ -147,6 +147,15 @@ int vt_cloud_load_tree(vt_cloud *cloud, const uint8_t *buf, size_t size, int upl
 int vt_slice(const vt_cloud *cloud, const double extent[4], size_t nx, size_t ny, double depth,
              double *d_out, void *stream);
 
+/* Declares that the rays of the cloud's coming ray calls are the pixels of
+ * row-major images whose rows hold `row` rays (0: no such order, the default).
+ * The kernels then take the rays of whole 8-row bands in 8x8-pixel tiles, so a
+ * wave's rays are image neighbours that walk the same part of the tree.  Speed
+ * only: every output is still written at its ray's own index, with the same
+ * bits.  No reference counterpart (the reference's OpenMP loop takes rays in
+ * index order, src/projection.cpp:43-68); grid_projection sets it. */
+int vt_cloud_set_ray_rows(vt_cloud *cloud, int64_t row);
+
 /* Replaces the cloud's fields, given on the device in filtered point order
  * (row i belongs to point vt_cloud_orig_ids()[i]; n x nfields, the nfields of
  * vt_cloud_create), without rebuilding the tree, and re-applies the RGBA

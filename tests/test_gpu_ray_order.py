@@ -1,0 +1,164 @@
+"""GPU tests of the 8x8-tile ray order (vt_cloud_set_ray_rows).
+
+The ray kernels may take image rays in 8x8-pixel tiles instead of index
+order; that changes only which lane walks which ray when.  Every output is
+written at its ray's own index, so each result must be bit-identical to the
+index-order run (itself pinned to the oracle elsewhere), and a ray that the
+order mapping skipped or took twice would show up as a zero or a wrong row.
+Row lengths cover whole tiles, a ragged right edge (row not a multiple of 8),
+a ragged last band (rays not a multiple of 8 rows), a row of one ray, and rows
+longer than the ray count (no whole band: index order).
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import LATTICE_RAYS, cubic_lattice
+from oracle import port
+
+pytestmark = pytest.mark.gpu
+
+import vortrace_amd as vt  # noqa: E402
+from vortrace_amd import grid, synthetic  # noqa: E402
+
+BB = [0, 100, 0, 100, 0, 100]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _image_rays(nx, ny, periodic):
+    s, e = grid.generate_projection_grid([[3.0, 97.0], [1.0, 99.0]], (nx, ny),
+                                         [0.0, 100.0], None)
+    return s.reshape(-1, 3), e.reshape(-1, 3)
+
+
+@pytest.fixture(scope="module")
+def clouds():
+    rng = np.random.default_rng(71)
+    pos, rho = synthetic.clustered_mesh(30_000, seed=72, n_clumps=10)
+    f2 = np.column_stack([rho, rng.random(len(pos))])
+    rgba = np.empty((len(pos), 4))
+    rgba[:, :3] = 0.1 + 0.8 * rng.random((len(pos), 3))
+    rgba[:, 3] = 0.02 + 0.05 * rng.random(len(pos))
+    out = {}
+    for periodic in (False, True):
+        out[("sum", periodic)] = (vt.ProjectionCloud(pos, f2, boundbox=BB, periodic=periodic),
+                                  pos, f2)
+        out[("volume", periodic)] = (vt.ProjectionCloud(pos, rgba, boundbox=BB,
+                                                        periodic=periodic), pos, rgba)
+    return out
+
+
+def _with_rows(pc, row, fn):
+    pc.set_ray_rows(row)
+    try:
+        return fn()
+    finally:
+        pc.set_ray_rows(0)
+
+
+SHAPES = [(40, 24, 24), (37, 13, 13), (16, 16, 1), (9, 31, 31), (12, 10, 2000), (64, 64, 8)]
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("reduction", ["integrate", "max", "min", "volume"])
+@pytest.mark.parametrize("nx,ny,row", SHAPES)
+def test_tiled_projection_same_bits(clouds, periodic, reduction, nx, ny, row):
+    _say(f"tiled projection {reduction} periodic={periodic} image {nx}x{ny} row={row}")
+    pc, _, _ = clouds[("volume" if reduction == "volume" else "sum", periodic)]
+    s, e = _image_rays(nx, ny, periodic)
+    dev = torch.device("cuda")
+    ds_, de_ = torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)
+    ref = pc.projection_tensor(ds_, de_, reduction=reduction)
+    got = _with_rows(pc, row, lambda: pc.projection_tensor(ds_, de_, reduction=reduction))
+    assert torch.equal(got, ref)
+    assert bool(torch.all(ref[:, 0] != 0))  # every ray crosses cells
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_tiled_matches_oracle(clouds, periodic):
+    _say(f"tiled projection vs oracle periodic={periodic}")
+    pc, pos, f2 = clouds[("sum", periodic)]
+    s, e = _image_rays(21, 19, periodic)
+    oc = port.OracleCloud(pos, f2, BB, periodic=periodic)
+    expect = oc.project(s, e, "sum")
+    dev = torch.device("cuda")
+    got = _with_rows(pc, 19, lambda: pc.projection_tensor(
+        torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev), reduction="integrate"))
+    np.testing.assert_array_equal(got.cpu().numpy(), expect)
+
+
+@pytest.mark.parametrize("nx,ny,row", [(40, 24, 24), (37, 13, 13)])
+def test_tiled_segments_same_bits(clouds, nx, ny, row):
+    _say(f"tiled segments image {nx}x{ny} row={row}")
+    pc, _, _ = clouds[("sum", True)]
+    s, e = _image_rays(nx, ny, True)
+    dev = torch.device("cuda")
+    ds_, de_ = torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)
+    ref = pc.segments_tensor(ds_, de_, reduction="integrate")
+    got = _with_rows(pc, row, lambda: pc.segments_tensor(ds_, de_, reduction="integrate"))
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("reduction", ["integrate", "volume"])
+def test_tiled_gradient_same_bits(clouds, reduction):
+    """Forward + backward twice (the second forward records its segments)."""
+    _say(f"tiled gradient {reduction}")
+    pc, _, f = clouds[("volume" if reduction == "volume" else "sum", True)]
+    s, e = _image_rays(32, 24, True)
+    dev = torch.device("cuda")
+    ds_, de_ = torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)
+    g = torch.from_numpy(np.random.default_rng(73).standard_normal(
+        (len(s), 3 if reduction == "volume" else f.shape[1]))).to(dev)
+
+    def grads():
+        res = []
+        for _ in range(2):
+            x = torch.from_numpy(f).to(dev).requires_grad_(True)
+            img = pc.projection_grad(ds_, de_, x, reduction=reduction)
+            (img * g).sum().backward()
+            res.append((img.detach().clone(), x.grad.clone()))
+        return res
+
+    ref = grads()
+    got = _with_rows(pc, 24, grads)
+    for (ia, ga), (ib, gb) in zip(got, ref):
+        assert torch.equal(ia, ib) and torch.equal(ga, gb)
+
+
+def test_tiled_deferred_rays():
+    """Rays that need the K_EXPAND search run in a second launch through a ray
+    list, which takes them in list order; the tiled first launch must hand it
+    the right ray indices."""
+    _say("tiled order with deferred rays")
+    pos, _, vol, bb = cubic_lattice()
+    f = 0.5 + np.random.default_rng(74).random(len(pos))
+    pc = vt.ProjectionCloud(pos, f, boundbox=bb, vol=vol)
+    s = np.array([r[0] for r in LATTICE_RAYS] * 40, float)
+    e = np.array([r[1] for r in LATTICE_RAYS] * 40, float)
+    dev = torch.device("cuda")
+    ds_, de_ = torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)
+    ref = pc.projection_tensor(ds_, de_, reduction="integrate")
+    got = _with_rows(pc, len(LATTICE_RAYS), lambda: pc.projection_tensor(
+        ds_, de_, reduction="integrate"))
+    deferred = vt._lib.last_call_stats().deferred_rays
+    _say(f"  {len(s)} rays, {deferred} deferred")
+    assert deferred > 0
+    assert torch.equal(got, ref)
+
+
+def test_grid_projection_restores_row(clouds):
+    """grid_projection takes its rays in tiles and leaves the cloud's setting
+    as it was; its image equals the index-order projection of the same rays."""
+    pc, _, _ = clouds[("sum", True)]
+    s, e = grid.generate_projection_grid([0.0, 100.0], 24, [0.0, 100.0], None)
+    img = pc.grid_projection([0.0, 100.0], 24, [0.0, 100.0], None)
+    assert pc._cloud.get_ray_rows() == 0
+    ref = pc.projection(s.reshape(-1, 3), e.reshape(-1, 3))
+    np.testing.assert_array_equal(img, ref.reshape(img.shape))

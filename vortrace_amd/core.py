@@ -223,6 +223,17 @@ class PointCloud:
         _lib.check(_lib.lib().vt_cloud_load_tree(self._handle(), buf, len(data), 0))
 
     # -- fields on the device ----------------------------------------------------
+    def set_ray_rows(self, row: int) -> None:
+        """Declares the coming calls' rays to be the pixels of row-major
+        images with `row` rays per row (0: no image order), so the kernels
+        take them in 8x8-pixel tiles (vt_cloud_set_ray_rows).  Speed only."""
+        row = int(row)
+        _lib.check(_lib.lib().vt_cloud_set_ray_rows(self._handle(), row))
+        self._ray_rows = row
+
+    def get_ray_rows(self) -> int:
+        return getattr(self, "_ray_rows", 0)
+
     def set_fields_tensor(self, fields_filtered: torch.Tensor) -> None:
         """Replaces the fields with a device tensor in filtered point order
         (n, nfields), keeping the tree (vt_cloud_set_fields)."""

@@ -104,6 +104,7 @@ struct Scratch {
   int leaf_batch;      // ray engine: lanes of a wave that run their leaf steps together
   int parts;           // ray engine: ray partitions (1 or kParts)
   int fetch_lanes;     // ray engine: lanes of each wave that take rays (1..64)
+  uint32_t tile_row;   // ray engine: rays per image row for the 8x8-tile ray order (0 = off)
 };
 
 constexpr int kMaxQueryK = 128;

@@ -417,6 +417,25 @@ enum : int {
   kPhDone = 3,   // no rays left
 };
 
+// The 8x8-tile ray order: with the rays the pixels of row-major images whose
+// rows hold w rays, fetch slot j is taken as the pixel (8b + r, 8t + c) of band
+// b (8 rows), column tile t (8 columns, fewer at the right edge) and row r of
+// the tile, so that the 64 rays a wave holds at once are neighbours on the
+// image and walk through the same part of the tree.  A bijection of [0, nfull)
+// (nfull = the rays of whole bands, at most nrays); slots from nfull on keep
+// their index.  Only the order rays are taken in changes, never a result.
+__device__ __forceinline__ uint32_t tile_ray(uint32_t j, uint32_t w, uint32_t nfull) {
+  if (j >= nfull) return j;
+  const uint32_t band = 8u * w;
+  const uint32_t b = j / band;
+  const uint32_t q = j - b * band;
+  const uint32_t t = q >> 6;
+  const uint32_t cw = min(8u, w - 8u * t);
+  const uint32_t u = q - (t << 6);
+  const uint32_t r = u / cw;
+  return (8u * b + r) * w + 8u * t + (u - r * cw);
+}
+
 struct EngineCounters {
   uint64_t nseg = 0, nq = 0, nslow = 0;
   int maxw = 0;
@@ -447,6 +466,9 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
   const int64_t psize = (nrays + nparts - 1) / nparts;
   int part = blockIdx.x & (nparts - 1);
   int part_tries = 0;
+  const uint32_t tile_w = (ray_list || nrays > 0xFFFFFFFFll) ? 0u : S.tile_row;
+  const uint32_t tile_n =
+      tile_w ? (uint32_t)((uint64_t)nrays / (8ull * tile_w) * (8ull * tile_w)) : 0u;
   for (;;) {
     {  // walk steps of every waiting lane
       WalkErr e{{0.0, 0.0, 0.0}, {0u, 0u}};
@@ -502,6 +524,7 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
           ph = kPhDone;
         } else {
           if (ray_list) ray = (int64_t)gld(ray_list + ray);
+          else if (tile_n) ray = (int64_t)tile_ray((uint32_t)ray, tile_w, tile_n);
           W.begin(start + 3 * ray, end + 3 * ray);
           io.begin(ray, sk);
           W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e);  // kStStart: the ray's start

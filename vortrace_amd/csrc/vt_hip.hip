@@ -572,6 +572,7 @@ struct vt_cloud {
   Stats *d_stats = nullptr;
   Scratch scratch{};
   size_t scratch_lanes = 0;
+  uint32_t ray_row = 0;         // rays per image row of the coming calls (0 = no image)
   uint32_t *d_defer = nullptr;  // rays left to the K_EXPAND-capable kernels
   size_t defer_cap = 0;
   int kspill = 0;
@@ -833,6 +834,7 @@ void launch(vt_cloud *c, K kernel, int64_t n, hipStream_t st, Timer &tm, A... ar
   const int64_t lanes = (int64_t)grid * kBlock;
   c->scratch.fetch_lanes =
       (int)std::min<int64_t>(64, std::max<int64_t>(1, (64 * n + lanes - 1) / lanes));
+  c->scratch.tile_row = c->ray_row;
   HIP_OK(hipEventRecord(tm.e0, st));
   kernel<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, args...);
   HIP_OK(hipGetLastError());
@@ -1568,6 +1570,13 @@ int vt_slice(const vt_cloud *cc, const double extent[4], size_t nx, size_t ny, d
       go(k_slice<false>);
     return finish_call(c, st, tm.e0, tm.e1);
   });
+}
+
+int vt_cloud_set_ray_rows(vt_cloud *c, int64_t row) {
+  if (!c) return fail(VT_EINVAL, "null cloud");
+  if (row < 0 || row > 0x7FFFFFFF) return fail(VT_EINVAL, "row length out of range [0, 2^31)");
+  c->ray_row = (uint32_t)row;
+  return VT_OK;
 }
 
 int vt_cloud_set_fields(vt_cloud *c, const double *d_fields, void *stream) {
