@@ -156,6 +156,16 @@ int vt_slice(const vt_cloud *cloud, const double extent[4], size_t nx, size_t ny
  * index order, src/projection.cpp:43-68); grid_projection sets it. */
 int vt_cloud_set_ray_rows(vt_cloud *cloud, int64_t row);
 
+/* Sets the order in which the kernels take the rays of the cloud's coming
+ * calls of exactly n rays: d_order (device, n entries) must be a permutation
+ * of [0, n) (checked; VT_EINVAL otherwise); the library keeps a copy.  NULL or
+ * n = 0 clears it.  Speed only: every output is still written at its ray's own
+ * index, with the same bits.  Fitting loops pass the rays longest first within
+ * each ray partition (from the previous pass's segment counts) so the last rays
+ * a lane takes are short.  No reference counterpart (src/projection.cpp:43-68
+ * takes rays in index order). */
+int vt_cloud_set_ray_order(vt_cloud *cloud, const uint32_t *d_order, int64_t n, void *stream);
+
 /* Replaces the cloud's fields, given on the device in filtered point order
  * (row i belongs to point vt_cloud_orig_ids()[i]; n x nfields, the nfields of
  * vt_cloud_create), without rebuilding the tree, and re-applies the RGBA

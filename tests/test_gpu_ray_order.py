@@ -162,3 +162,80 @@ def test_grid_projection_restores_row(clouds):
     assert pc._cloud.get_ray_rows() == 0
     ref = pc.projection(s.reshape(-1, 3), e.reshape(-1, 3))
     np.testing.assert_array_equal(img, ref.reshape(img.shape))
+
+
+def _perm(n, seed):
+    return torch.from_numpy(np.random.default_rng(seed).permutation(n).astype(np.int32))
+
+
+@pytest.mark.parametrize("reduction", ["integrate", "max", "volume"])
+def test_ray_order_same_bits(clouds, reduction):
+    """A caller's ray order (vt_cloud_set_ray_order) -- a random permutation
+    and the longest-first order -- gives the index-order bits; calls of another
+    ray count ignore it."""
+    from vortrace_amd.autograd import longest_first
+    _say(f"ray order {reduction}")
+    pc, _, _ = clouds[("volume" if reduction == "volume" else "sum", True)]
+    s, e = _image_rays(40, 24, True)
+    dev = torch.device("cuda")
+    ds_, de_ = torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)
+    n = len(s)
+    counts = torch.empty(n, dtype=torch.int64, device=dev)
+    mode = pc._validate_reduction(reduction)
+    ref = vt.Projection(ds_, de_).project_tensor(pc._cloud, mode, counts=counts)
+    for order in (_perm(n, 81).to(dev), longest_first(counts)):
+        pc.set_ray_order(order)
+        try:
+            got = pc.projection_tensor(ds_, de_, reduction=reduction)
+            assert torch.equal(got, ref)
+            segs_o = pc.segments_tensor(ds_, de_, reduction=reduction)
+            other = pc.projection_tensor(ds_[:-7], de_[:-7], reduction=reduction)
+        finally:
+            pc.set_ray_order(None)
+        segs = pc.segments_tensor(ds_, de_, reduction=reduction)
+        for a, b in zip(segs_o, segs):
+            assert torch.equal(a, b)
+        assert torch.equal(other, ref[:-7])
+
+
+def test_ray_order_rejects_non_permutations(clouds):
+    pc, _, _ = clouds[("sum", True)]
+    dev = torch.device("cuda")
+    s, e = _image_rays(8, 8, True)
+    ds_, de_ = torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)
+    ref = pc.projection_tensor(ds_, de_)
+    dup = _perm(64, 82)
+    dup[5] = dup[6]
+    big = _perm(64, 83)
+    big[0] = 64
+    for bad in (dup, big):
+        with pytest.raises(ValueError, match="permutation"):
+            pc.set_ray_order(bad.to(dev))
+        # a rejected order is not used
+        assert torch.equal(pc.projection_tensor(ds_, de_), ref)
+    with pytest.raises(ValueError):
+        pc.set_ray_order(_perm(64, 84))  # host tensor
+
+
+def test_longest_first_gradient_same_bits(clouds, monkeypatch):
+    """The autograd forward takes rays longest first from its second pass on
+    (here at any ray count): the image and gradient of every pass equal the
+    first (index-order) pass."""
+    import vortrace_amd.autograd as ag
+    monkeypatch.setattr(ag, "_order_pays", lambda n, device: True)
+    _say("longest-first gradient passes")
+    pos, rho = synthetic.clustered_mesh(20_000, seed=85, n_clumps=8)
+    pc = vt.ProjectionCloud(pos, rho, boundbox=BB, periodic=True)
+    s, e = _image_rays(48, 40, True)
+    dev = torch.device("cuda")
+    ds_, de_ = torch.from_numpy(s).to(dev), torch.from_numpy(e).to(dev)
+    g = torch.from_numpy(np.random.default_rng(86).standard_normal(len(s))).to(dev)
+    res = []
+    for _ in range(3):
+        x = torch.from_numpy(rho).to(dev).requires_grad_(True)
+        img = pc.projection_grad(ds_, de_, x, reduction="integrate")
+        (img * g).sum().backward()
+        res.append((img.detach().clone(), x.grad.clone()))
+    assert pc._longest_first is not None
+    for img, gr in res[1:]:
+        assert torch.equal(img, res[0][0]) and torch.equal(gr, res[0][1])

@@ -234,3 +234,60 @@ def test_synthetic_is_seeded_and_bounded():
     from scipy.spatial import cKDTree
     d, _ = cKDTree(p1, boxsize=100.0).query(p1, k=2)
     assert d[:, 1].min() >= 1e-3
+
+
+def _tile_order_loops(n, row):
+    """The 8x8-tile fetch order written as loops: bands of 8 rows, tiles of 8
+    columns (fewer at the right edge), rows of a tile, columns of a row; the
+    rays past the last whole band in index order."""
+    out = []
+    nbands = n // (8 * row) if row else 0
+    for b in range(nbands):
+        for t0 in range(0, row, 8):
+            for r in range(8):
+                for c in range(t0, min(t0 + 8, row)):
+                    out.append((8 * b + r) * row + c)
+    return out + list(range(len(out), n))
+
+
+@pytest.mark.parametrize("n,row", [(0, 5), (7, 0), (64, 8), (481, 13), (960, 24),
+                                   (279, 31), (120, 2000), (256, 1)])
+def test_tiled_slots(n, row):
+    from vortrace_amd.autograd import tiled_slots
+    assert tiled_slots(n, row).tolist() == _tile_order_loops(n, row)
+
+
+@pytest.mark.parametrize("n,row,parts,block", [(1, 0, 8, 4), (37, 0, 8, 128),
+                                               (1000, 0, 8, 64), (1000, 0, 1, 256),
+                                               (4096, 64, 8, 1 << 15), (4800, 24, 8, 192),
+                                               (999, 13, 8, 320)])
+def test_longest_first_order(n, row, parts, block):
+    """longest_first: a permutation of [0, n); cut the fetch order (tiles of
+    `row`-ray rows) in runs of 64 slots: every block of `block` slots of a ray
+    partition holds its own runs, whole, by descending largest count, slot
+    order among equals."""
+    import torch
+    from vortrace_amd.autograd import longest_first
+    from vortrace_amd import tuning
+    counts = torch.from_numpy(np.random.default_rng(n).integers(0, 40, n))
+    old = tuning.get_tuning()
+    try:
+        tuning.set_tuning(ray_parts=parts)
+        order = longest_first(counts, row, block)
+    finally:
+        tuning.set_tuning(**old)
+    o = order.numpy().astype(np.int64)
+    assert order.dtype == torch.int32
+    assert np.array_equal(np.sort(o), np.arange(n))
+    fetch = _tile_order_loops(n, row)
+    c = counts.numpy()
+    psize = -(-n // parts)
+    for p in range(parts):
+        for lo in range(p * psize, min(n, (p + 1) * psize), block):
+            hi = min(n, (p + 1) * psize, lo + block)
+            runs = {}
+            for sl in range(lo, hi):
+                runs.setdefault((sl - lo) // 64, []).append(sl)
+            ranked = sorted(runs, key=lambda k: (-max(c[fetch[sl]] for sl in runs[k]), k))
+            expect = [fetch[sl] for k in ranked for sl in runs[k]]
+            assert list(o[lo:hi]) == expect

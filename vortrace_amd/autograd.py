@@ -18,6 +18,12 @@ The pool is sized from the segment count of the previous forward pass on the
 same cloud; on the first pass, or when the pool overflows, the backward pass
 walks the rays for their segments (``vt_adjoint``).  Both give the same bits.
 
+From the second pass on, when the rays are one to four times the lanes the
+kernel keeps resident, the forward pass also takes them longest first (by the
+previous pass's segment counts, in whole runs of 64 neighbouring rays within
+blocks of each ray partition), which shortens the kernel's tail; the order
+never changes a result.
+
 The reference has no derivatives; the tree and the segments are the
 reference's, so the gradient is the exact derivative of its projection.
 Points that the cloud filtered out (non-periodic padding filter,
@@ -41,6 +47,76 @@ def _pool_capacity(estimate: int) -> int:
     return int(estimate * _POOL_GROWTH) + _POOL_SLACK
 
 
+# Rays per block of the longest-first order: 2^15 rays of an image are a band
+# of rows small enough that their cells stay in one XCD's L2.
+_ORDER_BLOCK = 1 << 15
+
+
+def _order_pays(n: int, device) -> bool:
+    """Whether the longest-first order is worth its loss of ray locality: only
+    when each resident lane takes more than one ray and at most four (about
+    512 lanes per CU are resident).  Measured on the forward kernel: 2 rays
+    per lane (512^2 rays, 2M clustered cells) -9 %; half a ray per lane
+    (256^2) +17 %; 32 rays per lane (16 views of 512^2) +1.5 %."""
+    lanes = 512 * torch.cuda.get_device_properties(device).multi_processor_count
+    return lanes < n <= 4 * lanes
+
+
+def tiled_slots(n: int, row: int, device=None) -> torch.Tensor:
+    """The ray each fetch slot takes under the 8x8-tile order of images with
+    `row` rays per row (vt_cloud_set_ray_rows; the identity for row 0 and
+    for the rays past the last whole band of 8 rows)."""
+    j = torch.arange(n, device=device, dtype=torch.int64)
+    band = 8 * row
+    nfull = (n // band) * band if row > 0 else 0
+    if nfull == 0:
+        return j
+    jf = j[:nfull]
+    b = jf // band
+    q = jf - b * band
+    t = q // 64
+    cw = torch.clamp(row - 8 * t, max=8)
+    u = q - 64 * t
+    r = u // cw
+    out = j.clone()
+    out[:nfull] = (8 * b + r) * row + 8 * t + (u - r * cw)
+    return out
+
+
+def longest_first(counts: torch.Tensor, row: int = 0,
+                  block: int = _ORDER_BLOCK) -> torch.Tensor | None:
+    """A ray order (int32 permutation of [0, n), for set_ray_order) from a
+    previous pass's segment counts.  The rays are first put in fetch order
+    (8x8 tiles of `row`-ray image rows, or index order for row 0); each ray
+    partition (the kernels' XCD partitions are contiguous ranges of slots) is
+    cut in blocks of `block` slots and each block in runs of 64 -- what one
+    wave takes at once; a block takes its runs whole, longest first (by the
+    run's largest count; slot order among equals).  The
+    rays a lane takes last are then short ones, and a wave's rays are still
+    neighbours.  None for 2^31 rays or more."""
+    from .tuning import get_tuning
+    n = counts.numel()
+    if n == 0 or n >= 2 ** 31:
+        return None
+    dev = counts.device
+    rays = tiled_slots(n, int(row), dev)
+    c = counts.to(torch.int64)[rays]
+    slot = torch.arange(n, device=dev, dtype=torch.int64)
+    parts = get_tuning()["ray_parts"]
+    psize = -(-n // parts)
+    per_part = -(-psize // block)
+    per_block = -(-block // 64)
+    part = slot // psize
+    pl = slot - part * psize
+    group = part * per_part + pl // block
+    run = group * per_block + (pl % block) // 64
+    run_max = torch.zeros(parts * per_part * per_block, dtype=torch.int64, device=dev)
+    run_max.scatter_reduce_(0, run, c, reduce="amax", include_self=False)
+    span = 1 << 40  # above any segment count
+    key = group * span + (span - 1 - run_max[run])
+    return rays[torch.argsort(key, stable=True)].to(torch.int32)
+
+
 class ProjectFields(torch.autograd.Function):
     """projection(fields) with the adjoint as its backward."""
 
@@ -52,17 +128,30 @@ class ProjectFields(torch.autograd.Function):
         counts = torch.empty(proj.ngrid, dtype=torch.int64, device=dev)
         records, n_records = None, -1
         estimate = getattr(pc, "_segment_estimate", None)
-        if ctx.needs_input_grad[0] and estimate is not None:
-            records = torch.empty((_pool_capacity(estimate), SEGREC_BYTES),
-                                  dtype=torch.uint8, device=dev)
-            out, n_records = proj.project_recorded_tensor(pc._cloud, mode, counts, records)
-            if n_records < 0:
-                records = None  # pool too small: the backward walks the rays
-        else:
-            out = proj.project_tensor(pc._cloud, mode, counts=counts)
-        pc._forward_stats = _lib.last_call_stats()
+        # The previous pass's counts order this pass's rays longest first
+        # (speed only; any permutation gives the same bits).
+        order = getattr(pc, "_longest_first", None)
+        ordered = order is not None and order.numel() == proj.ngrid
+        if ordered:
+            pc._cloud.set_ray_order(order)
+        try:
+            if ctx.needs_input_grad[0] and estimate is not None:
+                records = torch.empty((_pool_capacity(estimate), SEGREC_BYTES),
+                                      dtype=torch.uint8, device=dev)
+                out, n_records = proj.project_recorded_tensor(pc._cloud, mode, counts,
+                                                              records)
+                if n_records < 0:
+                    records = None  # pool too small: the backward walks the rays
+            else:
+                out = proj.project_tensor(pc._cloud, mode, counts=counts)
+            pc._forward_stats = _lib.last_call_stats()
+        finally:
+            if ordered:
+                pc._cloud.set_ray_order(None)
         if ctx.needs_input_grad[0]:
             pc._segment_estimate = int(counts.sum().item())
+            pc._longest_first = (longest_first(counts, pc._cloud.get_ray_rows())
+                                 if _order_pays(counts.numel(), dev) else None)
         ctx.save_for_backward(fields, counts)
         ctx.records, ctx.n_records = records, n_records
         ctx.pc, ctx.proj, ctx.mode = pc, proj, mode

@@ -234,6 +234,20 @@ class PointCloud:
     def get_ray_rows(self) -> int:
         return getattr(self, "_ray_rows", 0)
 
+    def set_ray_order(self, order: torch.Tensor | None) -> None:
+        """Order in which the kernels take the rays of coming calls with
+        len(order) rays: a permutation of [0, n) as an int32 device tensor,
+        or None to clear (vt_cloud_set_ray_order).  Speed only."""
+        if order is None:
+            _lib.check(_lib.lib().vt_cloud_set_ray_order(self._handle(), None, 0, _stream()))
+            return
+        if not isinstance(order, torch.Tensor) or order.dtype != torch.int32 \
+                or order.dim() != 1 or order.device != self._device:
+            raise ValueError("order must be a 1-D int32 tensor on the cloud's device")
+        order = order.contiguous()
+        _lib.check(_lib.lib().vt_cloud_set_ray_order(self._handle(), _P(order.data_ptr()),
+                                                     order.numel(), _stream()))
+
     def set_fields_tensor(self, fields_filtered: torch.Tensor) -> None:
         """Replaces the fields with a device tensor in filtered point order
         (n, nfields), keeping the tree (vt_cloud_set_fields)."""

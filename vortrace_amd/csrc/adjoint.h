@@ -223,6 +223,15 @@ __global__ void k_reduce_cells(uint32_t n, int nf, const uint32_t *start, const 
   }
 }
 
+// Marks a ray order that is not a permutation of [0, n): an entry out of
+// range, or one that an earlier entry already took.
+__global__ void k_check_order(int64_t n, const uint32_t *order, uint32_t *seen, int *bad) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t r = gld(order + j);
+  if ((int64_t)r >= n || atomicAdd(seen + r, 1u) != 0u) atomicOr(bad, 1);
+}
+
 __global__ void k_iota(int64_t n, uint32_t *v) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) gst(v + j, (uint32_t)j);

@@ -105,6 +105,7 @@ struct Scratch {
   int parts;           // ray engine: ray partitions (1 or kParts)
   int fetch_lanes;     // ray engine: lanes of each wave that take rays (1..64)
   uint32_t tile_row;   // ray engine: rays per image row for the 8x8-tile ray order (0 = off)
+  const uint32_t *ray_order;  // ray engine: the order fetch slots take rays in (null = own order)
 };
 
 constexpr int kMaxQueryK = 128;

@@ -466,6 +466,9 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
   const int64_t psize = (nrays + nparts - 1) / nparts;
   int part = blockIdx.x & (nparts - 1);
   int part_tries = 0;
+  // A caller's ray order (a permutation of [0, nrays), set only for launches
+  // of that many rays) stands in for the list of a launch that has none.
+  if (!ray_list) ray_list = S.ray_order;
   const uint32_t tile_w = (ray_list || nrays > 0xFFFFFFFFll) ? 0u : S.tile_row;
   const uint32_t tile_n =
       tile_w ? (uint32_t)((uint64_t)nrays / (8ull * tile_w) * (8ull * tile_w)) : 0u;

@@ -573,6 +573,9 @@ struct vt_cloud {
   Scratch scratch{};
   size_t scratch_lanes = 0;
   uint32_t ray_row = 0;         // rays per image row of the coming calls (0 = no image)
+  uint32_t *d_order = nullptr;  // a caller's ray order for calls of order_n rays
+  size_t order_cap = 0;
+  int64_t order_n = 0;
   uint32_t *d_defer = nullptr;  // rays left to the K_EXPAND-capable kernels
   size_t defer_cap = 0;
   int kspill = 0;
@@ -585,8 +588,12 @@ struct vt_cloud {
     for (void *p : {(void *)d_nodes, (void *)d_recs, (void *)d_gen, (void *)d_fields, (void *)d_slot_id,
                     (void *)d_err, (void *)d_stats, (void *)scratch.knn_spill,
                     (void *)scratch.walk_tag, (void *)scratch.walk_s,
-                    (void *)scratch.big_id, (void *)scratch.big_d, (void *)d_defer})
+                    (void *)scratch.big_id, (void *)scratch.big_d, (void *)d_defer,
+                    (void *)d_order})
       if (p) (void)hipFree(p);
+    d_order = nullptr;
+    order_cap = 0;
+    order_n = 0;
     for (auto &a : arena)
       if (a.first) (void)hipFree(a.first);
     arena.clear();
@@ -835,6 +842,7 @@ void launch(vt_cloud *c, K kernel, int64_t n, hipStream_t st, Timer &tm, A... ar
   c->scratch.fetch_lanes =
       (int)std::min<int64_t>(64, std::max<int64_t>(1, (64 * n + lanes - 1) / lanes));
   c->scratch.tile_row = c->ray_row;
+  c->scratch.ray_order = (c->order_n == n && n > 0) ? c->d_order : nullptr;
   HIP_OK(hipEventRecord(tm.e0, st));
   kernel<<<grid, kBlock, 0, st>>>(c->dc, c->scratch, args...);
   HIP_OK(hipGetLastError());
@@ -1577,6 +1585,40 @@ int vt_cloud_set_ray_rows(vt_cloud *c, int64_t row) {
   if (row < 0 || row > 0x7FFFFFFF) return fail(VT_EINVAL, "row length out of range [0, 2^31)");
   c->ray_row = (uint32_t)row;
   return VT_OK;
+}
+
+int vt_cloud_set_ray_order(vt_cloud *c, const uint32_t *d_order, int64_t n, void *stream) {
+  return guarded([&]() -> int {
+    if (!c) return fail(VT_EINVAL, "null cloud");
+    if (n < 0 || n > 0xFFFFFFFFll) return fail(VT_EINVAL, "ray count out of range [0, 2^32)");
+    c->order_n = 0;
+    if (n == 0 || !d_order) return VT_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (c->order_cap < (size_t)n) {
+      if (c->d_order) (void)hipFree(c->d_order);
+      c->d_order = nullptr;
+      c->order_cap = 0;
+      c->d_order = dev_alloc<uint32_t>((size_t)n);
+      c->order_cap = (size_t)n;
+    }
+    HIP_OK(hipMemcpyAsync(c->d_order, d_order, (size_t)n * sizeof(uint32_t),
+                          hipMemcpyDeviceToDevice, st));
+    DevBufs B(c);
+    uint32_t *seen = B.get<uint32_t>((size_t)n);
+    int *bad = B.get<int>(1);
+    HIP_OK(hipMemsetAsync(seen, 0, (size_t)n * sizeof(uint32_t), st));
+    HIP_OK(hipMemsetAsync(bad, 0, sizeof(int), st));
+    const int threads = 256;
+    k_check_order<<<(unsigned)((n + threads - 1) / threads), threads, 0, st>>>(n, c->d_order,
+                                                                               seen, bad);
+    HIP_OK(hipGetLastError());
+    int invalid = 0;
+    HIP_OK(hipMemcpyAsync(&invalid, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (invalid) return fail(VT_EINVAL, "ray order is not a permutation of [0, n)");
+    c->order_n = n;
+    return VT_OK;
+  });
 }
 
 int vt_cloud_set_fields(vt_cloud *c, const double *d_fields, void *stream) {

@@ -189,6 +189,14 @@ class ProjectionCloud:
         self._require_tree()
         self._cloud.set_ray_rows(row)
 
+    def set_ray_order(self, order: torch.Tensor | None) -> None:
+        """Sets the order in which the ray kernels take the rays of this
+        cloud's coming calls with len(order) rays: an int32 permutation of
+        [0, n) on the device, e.g. vortrace_amd.autograd.longest_first(counts)
+        from a previous pass's segment counts; None clears it.  Speed only."""
+        self._require_tree()
+        self._cloud.set_ray_order(order)
+
     @contextlib.contextmanager
     def _image_rows(self, row: int):
         prev = self._cloud.get_ray_rows()
