@@ -239,3 +239,27 @@ def test_longest_first_gradient_same_bits(clouds, monkeypatch):
     assert pc._longest_first is not None
     for img, gr in res[1:]:
         assert torch.equal(img, res[0][0]) and torch.equal(gr, res[0][1])
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_estimated_order_same_bits(periodic):
+    """projection_tensor orders rays by estimated cost when there are 1-4
+    rays per resident lane: the image equals the unordered projection's."""
+    _say(f"estimated ray order periodic={periodic}")
+    pos, rho = synthetic.clustered_mesh(20_000, seed=91, n_clumps=6)
+    pc = vt.ProjectionCloud(pos, rho, boundbox=BB, periodic=periodic)
+    dev = torch.device("cuda")
+    lanes = 512 * torch.cuda.get_device_properties(dev).multi_processor_count
+    npix = int(np.ceil(np.sqrt(1.5 * lanes)))
+    s, e = grid.generate_projection_grid([0.0, 100.0], npix, [0.0, 100.0], None)
+    ds_ = torch.from_numpy(s.reshape(-1, 3)).to(dev)
+    de_ = torch.from_numpy(e.reshape(-1, 3)).to(dev)
+    ref = vt.Projection(ds_, de_).project_tensor(pc._cloud, vt.ReductionMode.Sum)
+    assert pc._estimated_order(vt.Projection(ds_, de_)) is not None
+    pc.set_ray_rows(npix)
+    try:
+        got = pc.projection_tensor(ds_, de_)
+    finally:
+        pc.set_ray_rows(0)
+    assert pc._cloud.get_ray_order_size() == 0
+    assert torch.equal(got, ref)

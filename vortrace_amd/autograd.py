@@ -131,7 +131,8 @@ class ProjectFields(torch.autograd.Function):
         # The previous pass's counts order this pass's rays longest first
         # (speed only; any permutation gives the same bits).
         order = getattr(pc, "_longest_first", None)
-        ordered = order is not None and order.numel() == proj.ngrid
+        ordered = (order is not None and order.numel() == proj.ngrid
+                   and not pc._cloud.get_ray_order_size())
         if ordered:
             pc._cloud.set_ray_order(order)
         try:

@@ -240,13 +240,20 @@ class PointCloud:
         or None to clear (vt_cloud_set_ray_order).  Speed only."""
         if order is None:
             _lib.check(_lib.lib().vt_cloud_set_ray_order(self._handle(), None, 0, _stream()))
+            self._ray_order_n = 0
             return
         if not isinstance(order, torch.Tensor) or order.dtype != torch.int32 \
                 or order.dim() != 1 or order.device != self._device:
             raise ValueError("order must be a 1-D int32 tensor on the cloud's device")
         order = order.contiguous()
+        self._ray_order_n = 0  # a rejected order clears the previous one
         _lib.check(_lib.lib().vt_cloud_set_ray_order(self._handle(), _P(order.data_ptr()),
                                                      order.numel(), _stream()))
+        self._ray_order_n = order.numel()
+
+    def get_ray_order_size(self) -> int:
+        """Ray count of the order set by set_ray_order (0: none)."""
+        return getattr(self, "_ray_order_n", 0)
 
     def set_fields_tensor(self, fields_filtered: torch.Tensor) -> None:
         """Replaces the fields with a device tensor in filtered point order

@@ -1,0 +1,55 @@
+"""Ray orders for the plain forward projection.
+
+The ray kernels' time has a tail: the last rays handed out finish long after
+the rest when they are long ones.  A forward pass of a fitting loop orders its
+rays from the previous pass's segment counts (``autograd.longest_first``); a
+plain projection has no counts before it runs, so it orders them by an
+estimate: the cells a ray crosses grow with the cube root of the point
+density along it, so each ray's cost is estimated as its length times the
+mean of (points per cell)^(1/3) of a coarse G^3 histogram of the points,
+sampled at S points along the ray.  The order only decides which lane walks
+which ray when; every output keeps its ray's index and its bits.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+_G = 32        # histogram cells per axis
+_SAMPLES = 32  # samples along each ray
+
+
+def density_grid(pc, device) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """(points per cell)^(1/3) on a G^3 grid over the cloud's bounding box,
+    as a flat float32 device tensor, with the box's low corner and size."""
+    key = ("_cost_grid", str(device))
+    cached = getattr(pc, "_cost_grid", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    bb = np.asarray(pc.boundbox, dtype=np.float64).reshape(3, 2)
+    lo, size = bb[:, 0], np.maximum(bb[:, 1] - bb[:, 0], 1e-300)
+    p = pc.pos_orig[pc.orig_ids] if len(pc.orig_ids) != len(pc.pos_orig) else pc.pos_orig
+    ijk = np.clip(((p - lo) / size * _G).astype(np.int64), 0, _G - 1)
+    flat = (ijk[:, 0] * _G + ijk[:, 1]) * _G + ijk[:, 2]
+    h = np.bincount(flat, minlength=_G ** 3).astype(np.float32) ** (1.0 / 3.0)
+    out = (torch.from_numpy(h).to(device),
+           torch.tensor(lo, dtype=torch.float32, device=device),
+           torch.tensor(size, dtype=torch.float32, device=device))
+    pc._cost_grid = (key, out)
+    return out
+
+
+def estimated_order(pc, start: torch.Tensor, end: torch.Tensor, row: int = 0):
+    """autograd.longest_first of the estimated ray costs: an int32
+    permutation of the rays for set_ray_order."""
+    from .autograd import longest_first
+    grid, lo, size = density_grid(pc, start.device)
+    s = start.float()
+    d = end.float() - s
+    t = (torch.arange(_SAMPLES, device=start.device, dtype=torch.float32) + 0.5) / _SAMPLES
+    p = s[:, None, :] + t[None, :, None] * d[:, None, :]
+    ijk = torch.floor((p - lo) / size * _G).long()
+    ijk = torch.remainder(ijk, _G) if pc.periodic else ijk.clamp_(0, _G - 1)
+    flat = (ijk[..., 0] * _G + ijk[..., 1]) * _G + ijk[..., 2]
+    cost = grid[flat].sum(dim=1) * torch.linalg.vector_norm(d, dim=1)
+    return longest_first((cost * 1024.0).long(), row)

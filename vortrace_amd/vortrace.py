@@ -212,7 +212,26 @@ class ProjectionCloud:
         returns (N, n_out) FP64."""
         mode = self._validate_reduction(reduction)
         self._require_tree()
-        return Projection(pos_start, pos_end).project_tensor(self._cloud, mode)
+        proj = Projection(pos_start, pos_end)
+        order = self._estimated_order(proj)
+        if order is None:
+            return proj.project_tensor(self._cloud, mode)
+        self._cloud.set_ray_order(order)
+        try:
+            return proj.project_tensor(self._cloud, mode)
+        finally:
+            self._cloud.set_ray_order(None)
+
+    def _estimated_order(self, proj):
+        """Longest-first order of the rays by estimated cost (order.py) when it
+        pays (autograd._order_pays) and the caller set no order of its own."""
+        from .autograd import _order_pays
+        from .order import estimated_order
+        dev = self._cloud._device
+        if self._cloud.get_ray_order_size() or not _order_pays(proj.ngrid, dev):
+            return None
+        s, e = proj._rays(self._cloud)
+        return estimated_order(self, s, e, self._cloud.get_ray_rows())
 
     def batch_projection_tensor(self, views, *, reduction: str = "integrate"):
         """Projects several ray sets -- views of any sizes and directions,
