@@ -263,3 +263,21 @@ def test_estimated_order_same_bits(periodic):
         pc.set_ray_rows(0)
     assert pc._cloud.get_ray_order_size() == 0
     assert torch.equal(got, ref)
+
+
+def test_grid_projection_estimated_order():
+    """grid_projection at 1-4 rays per resident lane (tiles + estimated order)
+    equals the unordered projection of the same rays."""
+    pos, rho = synthetic.clustered_mesh(20_000, seed=92, n_clumps=6)
+    pc = vt.ProjectionCloud(pos, rho, boundbox=BB, periodic=True)
+    dev = torch.device("cuda")
+    lanes = 512 * torch.cuda.get_device_properties(dev).multi_processor_count
+    npix = int(np.ceil(np.sqrt(2.5 * lanes)))
+    _say(f"grid_projection {npix}^2 with the estimated order")
+    img = pc.grid_projection([0.0, 100.0], npix, [0.0, 100.0], None)
+    s, e = grid.generate_projection_grid([0.0, 100.0], npix, [0.0, 100.0], None)
+    ref = vt.Projection(torch.from_numpy(s.reshape(-1, 3)).to(dev),
+                        torch.from_numpy(e.reshape(-1, 3)).to(dev)).project_tensor(
+                            pc._cloud, vt.ReductionMode.Sum)
+    assert pc._cloud.get_ray_order_size() == 0 and pc._cloud.get_ray_rows() == 0
+    np.testing.assert_array_equal(img, ref.cpu().numpy().reshape(img.shape))

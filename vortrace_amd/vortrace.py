@@ -105,7 +105,15 @@ class ProjectionCloud:
         p = Projection(start.reshape(-1, 3), end.reshape(-1, 3))
         self._require_tree()
         with self._image_rows(shape[1]):
-            p.makeProjection(self._cloud, mode)
+            order = self._estimated_order(p)
+            if order is None:
+                p.makeProjection(self._cloud, mode)
+            else:
+                self._cloud.set_ray_order(order)
+                try:
+                    p.makeProjection(self._cloud, mode)
+                finally:
+                    self._cloud.set_ray_order(None)
         dat = p.returnProjection()
         nout = self._out_fields(mode)
         if nout == 1:
