@@ -295,8 +295,8 @@ def test_longest_first_order(n, row, parts, block):
 
 def test_estimated_order_is_a_permutation_led_by_dense_rays():
     """order.estimated_order (on CPU tensors): a permutation of the rays, also
-    for degenerate and out-of-box rays; the run of 64 rays holding the ray
-    through the densest cells goes first in its ray partition."""
+    for degenerate and out-of-box rays; the run of 64 rays through the
+    densest cells goes first in its ray partition."""
     import torch
     from vortrace_amd import tuning
     from vortrace_amd.order import estimated_order
@@ -313,7 +313,7 @@ def test_estimated_order_is_a_permutation_led_by_dense_rays():
     pc.orig_ids = np.arange(len(pts))
     n = 2048  # 8 partitions of 4 runs
     s = np.column_stack([rng.uniform(40, 100, n), rng.uniform(0, 50, n), np.zeros(n)])
-    s[200] = (20.0, 70.0, 0.0)     # through the clump: partition 0, run 3
+    s[192:256] = (20.0, 70.0, 0.0)  # through the clump: partition 0, run 3
     s[8] = (150.0, -30.0, 0.0)     # outside the box
     e = s.copy()
     e[:, 2] = 100.0

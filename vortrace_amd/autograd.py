@@ -106,15 +106,25 @@ def longest_first(counts: torch.Tensor, row: int = 0,
     psize = -(-n // parts)
     per_part = -(-psize // block)
     per_block = -(-block // 64)
+    nrun = parts * per_part * per_block
     part = slot // psize
     pl = slot - part * psize
-    group = part * per_part + pl // block
-    run = group * per_block + (pl % block) // 64
-    run_max = torch.zeros(parts * per_part * per_block, dtype=torch.int64, device=dev)
+    run = (part * per_part + pl // block) * per_block + (pl % block) // 64
+    run_max = torch.zeros(nrun, dtype=torch.int64, device=dev)
     run_max.scatter_reduce_(0, run, c, reduce="amax", include_self=False)
+    # Run ids grow with the slot, so a run's slots start at the exclusive sum
+    # of the earlier runs' lengths; sort the runs (not the rays) and lay them
+    # out again in that order.
+    run_len = torch.bincount(run, minlength=nrun)
+    run_start = torch.cumsum(run_len, 0) - run_len
     span = 1 << 40  # above any segment count
-    key = group * span + (span - 1 - run_max[run])
-    return rays[torch.argsort(key, stable=True)].to(torch.int32)
+    group = torch.arange(nrun, device=dev, dtype=torch.int64) // per_block
+    perm = torch.argsort(group * span + (span - 1 - run_max), stable=True)
+    plen = run_len[perm]
+    out_start = torch.cumsum(plen, 0) - plen
+    at = torch.repeat_interleave(perm, plen, output_size=n)
+    within = slot - torch.repeat_interleave(out_start, plen, output_size=n)
+    return rays[run_start[at] + within].to(torch.int32)
 
 
 class ProjectFields(torch.autograd.Function):

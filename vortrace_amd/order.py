@@ -7,7 +7,7 @@ plain projection has no counts before it runs, so it orders them by an
 estimate: the cells a ray crosses grow with the cube root of the point
 density along it, so each ray's cost is estimated as its length times the
 mean of (points per cell)^(1/3) of a coarse G^3 histogram of the points,
-sampled at S points along the ray.  The order only decides which lane walks
+sampled at S points along the ray (one ray per run of 64 neighbours).  The order only decides which lane walks
 which ray when; every output keeps its ray's index and its bits.
 """
 from __future__ import annotations
@@ -39,10 +39,9 @@ def density_grid(pc, device) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     return out
 
 
-def estimated_order(pc, start: torch.Tensor, end: torch.Tensor, row: int = 0):
-    """autograd.longest_first of the estimated ray costs: an int32
-    permutation of the rays for set_ray_order."""
-    from .autograd import longest_first
+def ray_costs(pc, start: torch.Tensor, end: torch.Tensor) -> torch.Tensor:
+    """Estimated cost of each ray (float32): its length times the sum of
+    (points per cell)^(1/3) at S points along it."""
     grid, lo, size = density_grid(pc, start.device)
     s = start.float()
     d = end.float() - s
@@ -51,5 +50,21 @@ def estimated_order(pc, start: torch.Tensor, end: torch.Tensor, row: int = 0):
     ijk = torch.floor((p - lo) / size * _G).long()
     ijk = torch.remainder(ijk, _G) if pc.periodic else ijk.clamp_(0, _G - 1)
     flat = (ijk[..., 0] * _G + ijk[..., 1]) * _G + ijk[..., 2]
-    cost = grid[flat].sum(dim=1) * torch.linalg.vector_norm(d, dim=1)
-    return longest_first((cost * 1024.0).long(), row)
+    return grid[flat].sum(dim=1) * torch.linalg.vector_norm(d, dim=1)
+
+
+def estimated_order(pc, start: torch.Tensor, end: torch.Tensor, row: int = 0):
+    """autograd.longest_first of estimated ray costs: an int32 permutation
+    of the rays for set_ray_order.  The order moves whole runs of 64 rays in
+    fetch order (an 8x8 tile of an image), whose rays share histogram cells,
+    so only the middle ray of each run of 64 fetch slots is estimated and
+    stands for the run."""
+    from .autograd import longest_first, tiled_slots
+    n = start.shape[0]
+    dev = start.device
+    rays = tiled_slots(n, int(row), dev)
+    probe = rays[torch.clamp(torch.arange(0, n, 64, device=dev) + 32, max=n - 1)]
+    cost = (ray_costs(pc, start[probe], end[probe]) * 1024.0).long()
+    counts = torch.empty(n, dtype=torch.int64, device=dev)
+    counts[rays] = torch.repeat_interleave(cost, 64)[:n]
+    return longest_first(counts, row)
