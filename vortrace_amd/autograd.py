@@ -83,8 +83,8 @@ def tiled_slots(n: int, row: int, device=None) -> torch.Tensor:
     return out
 
 
-def longest_first(counts: torch.Tensor, row: int = 0,
-                  block: int = _ORDER_BLOCK) -> torch.Tensor | None:
+def longest_first(counts: torch.Tensor, row: int = 0, block: int = _ORDER_BLOCK,
+                  rays: torch.Tensor | None = None) -> torch.Tensor | None:
     """A ray order (int32 permutation of [0, n), for set_ray_order) from a
     previous pass's segment counts.  The rays are first put in fetch order
     (8x8 tiles of `row`-ray image rows, or index order for row 0); each ray
@@ -99,7 +99,8 @@ def longest_first(counts: torch.Tensor, row: int = 0,
     if n == 0 or n >= 2 ** 31:
         return None
     dev = counts.device
-    rays = tiled_slots(n, int(row), dev)
+    if rays is None:  # callers that have it pass tiled_slots(n, row)
+        rays = tiled_slots(n, int(row), dev)
     c = counts.to(torch.int64)[rays]
     slot = torch.arange(n, device=dev, dtype=torch.int64)
     parts = get_tuning()["ray_parts"]

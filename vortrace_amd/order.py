@@ -67,4 +67,4 @@ def estimated_order(pc, start: torch.Tensor, end: torch.Tensor, row: int = 0):
     cost = (ray_costs(pc, start[probe], end[probe]) * 1024.0).long()
     counts = torch.empty(n, dtype=torch.int64, device=dev)
     counts[rays] = torch.repeat_interleave(cost, 64)[:n]
-    return longest_first(counts, row)
+    return longest_first(counts, row, rays=rays)
