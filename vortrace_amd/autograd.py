@@ -31,6 +31,8 @@ src/pointcloud.cpp:55-108) get a zero gradient.
 """
 from __future__ import annotations
 
+import functools
+
 import torch
 
 from . import _lib
@@ -83,8 +85,30 @@ def tiled_slots(n: int, row: int, device=None) -> torch.Tensor:
     return out
 
 
-def longest_first(counts: torch.Tensor, row: int = 0, block: int = _ORDER_BLOCK,
-                  rays: torch.Tensor | None = None) -> torch.Tensor | None:
+@functools.lru_cache(maxsize=4)
+def order_plan(n: int, row: int, parts: int, block: int, device: torch.device):
+    """The parts of longest_first that depend only on the sizes: the fetch
+    order (tiled_slots), each slot's run, the runs' first slots and lengths,
+    and each run's block (its sort group)."""
+    rays = tiled_slots(n, row, device)
+    slot = torch.arange(n, device=device, dtype=torch.int64)
+    psize = -(-n // parts)
+    per_part = -(-psize // block)
+    per_block = -(-block // 64)
+    nrun = parts * per_part * per_block
+    part = slot // psize
+    pl = slot - part * psize
+    run = (part * per_part + pl // block) * per_block + (pl % block) // 64
+    # Run ids grow with the slot, so a run's slots start at the exclusive sum
+    # of the earlier runs' lengths.
+    run_len = torch.bincount(run, minlength=nrun)
+    run_start = torch.cumsum(run_len, 0) - run_len
+    group = torch.arange(nrun, device=device, dtype=torch.int64) // per_block
+    return rays, slot, run, run_len, run_start, group
+
+
+def longest_first(counts: torch.Tensor, row: int = 0,
+                  block: int = _ORDER_BLOCK) -> torch.Tensor | None:
     """A ray order (int32 permutation of [0, n), for set_ray_order) from a
     previous pass's segment counts.  The rays are first put in fetch order
     (8x8 tiles of `row`-ray image rows, or index order for row 0); each ray
@@ -99,27 +123,13 @@ def longest_first(counts: torch.Tensor, row: int = 0, block: int = _ORDER_BLOCK,
     if n == 0 or n >= 2 ** 31:
         return None
     dev = counts.device
-    if rays is None:  # callers that have it pass tiled_slots(n, row)
-        rays = tiled_slots(n, int(row), dev)
-    c = counts.to(torch.int64)[rays]
-    slot = torch.arange(n, device=dev, dtype=torch.int64)
-    parts = get_tuning()["ray_parts"]
-    psize = -(-n // parts)
-    per_part = -(-psize // block)
-    per_block = -(-block // 64)
-    nrun = parts * per_part * per_block
-    part = slot // psize
-    pl = slot - part * psize
-    run = (part * per_part + pl // block) * per_block + (pl % block) // 64
-    run_max = torch.zeros(nrun, dtype=torch.int64, device=dev)
-    run_max.scatter_reduce_(0, run, c, reduce="amax", include_self=False)
-    # Run ids grow with the slot, so a run's slots start at the exclusive sum
-    # of the earlier runs' lengths; sort the runs (not the rays) and lay them
-    # out again in that order.
-    run_len = torch.bincount(run, minlength=nrun)
-    run_start = torch.cumsum(run_len, 0) - run_len
+    rays, slot, run, run_len, run_start, group = order_plan(
+        n, int(row), get_tuning()["ray_parts"], int(block), dev)
+    run_max = torch.zeros(run_len.numel(), dtype=torch.int64, device=dev)
+    run_max.scatter_reduce_(0, run, counts.to(torch.int64)[rays], reduce="amax",
+                            include_self=False)
+    # Sort the runs (not the rays) and lay them out again in that order.
     span = 1 << 40  # above any segment count
-    group = torch.arange(nrun, device=dev, dtype=torch.int64) // per_block
     perm = torch.argsort(group * span + (span - 1 - run_max), stable=True)
     plen = run_len[perm]
     out_start = torch.cumsum(plen, 0) - plen

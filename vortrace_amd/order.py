@@ -59,12 +59,13 @@ def estimated_order(pc, start: torch.Tensor, end: torch.Tensor, row: int = 0):
     fetch order (an 8x8 tile of an image), whose rays share histogram cells,
     so only the middle ray of each run of 64 fetch slots is estimated and
     stands for the run."""
-    from .autograd import longest_first, tiled_slots
+    from .autograd import _ORDER_BLOCK, longest_first, order_plan
+    from .tuning import get_tuning
     n = start.shape[0]
     dev = start.device
-    rays = tiled_slots(n, int(row), dev)
+    rays = order_plan(n, int(row), get_tuning()["ray_parts"], _ORDER_BLOCK, dev)[0]
     probe = rays[torch.clamp(torch.arange(0, n, 64, device=dev) + 32, max=n - 1)]
     cost = (ray_costs(pc, start[probe], end[probe]) * 1024.0).long()
     counts = torch.empty(n, dtype=torch.int64, device=dev)
     counts[rays] = torch.repeat_interleave(cost, 64)[:n]
-    return longest_first(counts, row, rays=rays)
+    return longest_first(counts, row)
