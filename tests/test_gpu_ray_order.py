@@ -243,7 +243,7 @@ def test_longest_first_gradient_same_bits(clouds, monkeypatch):
 
 @pytest.mark.parametrize("periodic", [False, True])
 def test_estimated_order_same_bits(periodic):
-    """projection_tensor orders rays by estimated cost when there are 1-4
+    """projection_tensor orders rays by estimated cost when there are 1-8
     rays per resident lane: the image equals the unordered projection's."""
     _say(f"estimated ray order periodic={periodic}")
     pos, rho = synthetic.clustered_mesh(20_000, seed=91, n_clumps=6)
@@ -266,7 +266,7 @@ def test_estimated_order_same_bits(periodic):
 
 
 def test_grid_projection_estimated_order():
-    """grid_projection at 1-4 rays per resident lane (tiles + estimated order)
+    """grid_projection at 1-8 rays per resident lane (tiles + estimated order)
     equals the unordered projection of the same rays."""
     pos, rho = synthetic.clustered_mesh(20_000, seed=92, n_clumps=6)
     pc = vt.ProjectionCloud(pos, rho, boundbox=BB, periodic=True)

@@ -18,7 +18,7 @@ The pool is sized from the segment count of the previous forward pass on the
 same cloud; on the first pass, or when the pool overflows, the backward pass
 walks the rays for their segments (``vt_adjoint``).  Both give the same bits.
 
-From the second pass on, when the rays are one to four times the lanes the
+From the second pass on, when the rays are one to eight times the lanes the
 kernel keeps resident, the forward pass also takes them longest first (by the
 previous pass's segment counts, in whole runs of 64 neighbouring rays within
 blocks of each ray partition), which shortens the kernel's tail; the order
@@ -56,12 +56,13 @@ _ORDER_BLOCK = 1 << 15
 
 def _order_pays(n: int, device) -> bool:
     """Whether the longest-first order is worth its loss of ray locality: only
-    when each resident lane takes more than one ray and at most four (about
+    when each resident lane takes more than one ray and at most eight (about
     512 lanes per CU are resident).  Measured on the forward kernel: 2 rays
-    per lane (512^2 rays, 2M clustered cells) -9 %; half a ray per lane
-    (256^2) +17 %; 32 rays per lane (16 views of 512^2) +1.5 %."""
+    per lane (512^2 rays, 2M clustered cells) -9 %; 8 rays per lane (1024^2,
+    volume) -2.5 %; half a ray per lane (256^2) +17 %; 32 rays per lane (16
+    views of 512^2) +1.5 %."""
     lanes = 512 * torch.cuda.get_device_properties(device).multi_processor_count
-    return lanes < n <= 4 * lanes
+    return lanes < n <= 8 * lanes
 
 
 def tiled_slots(n: int, row: int, device=None) -> torch.Tensor:
