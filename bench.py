@@ -96,6 +96,23 @@ def make_workload(name):
     return pos, fields, views, params
 
 
+def volume_fields(pos, rho):
+    """RGBA fields of the volume workloads (periodic box of side 100): kappa
+    such that the median opacity of a 64^2 z-probe is 0.5, then the
+    transfer function.  Returns (rgba (N, 4), kappa)."""
+    import torch
+    import vortrace_amd as vt
+    from vortrace_amd import synthetic
+    pc_probe = vt.ProjectionCloud(pos, rho, boundbox=[0, 100, 0, 100, 0, 100], periodic=True)
+    ps, pe = vt.grid.generate_projection_grid([0.0, 100.0], 64, [0.0, 100.0], None)
+    _, ids, se, sx, off = pc_probe.segments_tensor(
+        torch.from_numpy(ps.reshape(-1, 3)), torch.from_numpy(pe.reshape(-1, 3)))
+    kappa = synthetic.kappa_for_median_opacity(
+        rho, ids.cpu().numpy(), (sx - se).cpu().numpy(), off.cpu().numpy())
+    del pc_probe
+    return synthetic.rgba_transfer(rho, kappa), kappa
+
+
 def algorithmic_bytes(R, C, n_points, n_f, n_out):
     """R(48 + 8 n_out) + C(8 n_f) + 2C(32 D + 240), D = ceil(log2(N/10)):
     ray endpoints and outputs, one field gather per crossing, and two kNN
@@ -105,46 +122,159 @@ def algorithmic_bytes(R, C, n_points, n_f, n_out):
     return R * (48 + 8 * n_out) + C * (8 * n_f) + 2 * C * (32 * D + 240)
 
 
+def adjoint_bytes(R, C, n_points, n_f, n_out, sort_passes):
+    """Algorithmic bytes the one-walk adjoint adds to the forward's: per
+    segment the 32-byte record written by the forward and read back (64), its
+    (slot, ds) written in ray order (12), the contribution step (ds read 8,
+    contributions written 8 n_c; for volume also slot 4 and the cell's 4
+    fields 32 read, twice), the stable radix sort of (slot, index) pairs
+    (16 bytes per pass), run bounds (4) and the per-cell sums (index 4 and
+    contribution 8 n_c); per ray the count written, scanned and read (24)
+    and the image gradient read (8 n_out); per cell the gradient written
+    (8 n_f) with its run bounds and slot id (12).  n_c = n_f for sum, 4 for
+    volume."""
+    vol = n_out == 3
+    n_c = 4 if vol else n_f
+    per_seg = (64 + 12 + 8 + 8 * n_c + (2 * (4 + 32) if vol else 0) + 16 * sort_passes + 4
+               + 4 + 8 * n_c)
+    return C * per_seg + R * (24 + 8 * n_out) + n_points * (8 * n_f + 12)
+
+
+def sort_passes(n_points):
+    """Digit passes of the adjoint's radix sort of slots (8-bit digits)."""
+    return max(1, math.ceil(max(1, math.ceil(math.log2(max(2, n_points)))) / 8))
+
+
+def source_digest():
+    """SHA-256 (16 hex digits) of the HIP library's sources: ties a recorded
+    profile to the kernels it measured."""
+    import hashlib
+    d = os.path.join(ROOT, "vortrace_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".h", ".cpp")):
+            with open(os.path.join(d, name), "rb") as f:
+                h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
 def read_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE runs, scaled by a copy-kernel calibration) and the L2 hit rate
-    (TCC_HIT / (TCC_HIT + TCC_MISS)) of the same kernel, if recorded."""
+    (TCC_HIT / (TCC_HIT + TCC_MISS)) of the same kernel, as recorded in
+    profiles/traffic_<workload>.json, with the kernel time of the profiled run
+    and whether the recorded sources are the current ones."""
     p = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if not os.path.exists(p):
-        return None, None
+        return None
     with open(p) as f:
         d = json.load(f)
-    return d.get("hbm_bytes_per_launch"), d.get("l2_hit_rate")
+    return {"hbm_bytes_per_launch": d.get("hbm_bytes_per_launch"),
+            "l2_hit_rate": d.get("l2_hit_rate"),
+            "profiled_kernel_ms": d.get("kernel_ms"),
+            "profiled_source_digest": d.get("source_digest"),
+            "matches_current_source": d.get("source_digest") == source_digest()}
+
+
+def stream_copy_gbs(gib=2.0, repeats=5):
+    """Measured device copy bandwidth (read + write bytes / time) of a
+    `gib`-GiB FP64 copy, best of `repeats`, timed with HIP events."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    best = None
+    for _ in range(repeats):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 8 * n / (best * 1e-3) / 1e9
+
+
+def host_cpu_info():
+    """CPU model, sockets, cores per socket and threads per core (lscpu), and
+    the CPUs this process may run on."""
+    info = {"model": None, "sockets": None, "cores_per_socket": None,
+            "threads_per_core": None}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            v = v.strip()
+            if k == "Model name":
+                info["model"] = v
+            elif k == "Socket(s)":
+                info["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                info["cores_per_socket"] = int(v)
+            elif k == "Thread(s) per core":
+                info["threads_per_core"] = int(v)
+    except (OSError, ValueError):
+        pass
+    info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    if info["sockets"] and info["cores_per_socket"]:
+        info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    info["smt"] = (info["threads_per_core"] or 1) > 1
+    return info
 
 
 def cpu_baseline(pos, fields, start, end, periodic, mode, frac, gpu_vals):
-    """The reference's C++ core (oracle/_ref) on a strided sample of the rays."""
+    """The reference's C++ core (oracle/_ref; the oracle's restatement where
+    the reference build is absent) on a strided sample of the rays, in a
+    child process with OMP_NUM_THREADS = the physical cores available (at most
+    the box's CPU share, OMP_NUM_THREADS of this environment),
+    OMP_PROC_BIND=close and OMP_PLACES=cores; median of 3 timed runs."""
+    import subprocess
+    import tempfile
     from oracle import ref
-    if not ref.available():
-        return None
+    kind = "reference" if ref.available() else "port"
     stride = max(1, int(round(1.0 / frac)))
     idx = np.arange(0, len(start), stride)
-    c = ref.RefCloud(pos, fields, [0, 100, 0, 100, 0, 100], periodic=periodic)
-    c.project(start[idx[:64]], end[idx[:64]], mode)  # warm
-    t0 = time.perf_counter()
-    vals, ids, _, _, off = c.detailed(start[idx], end[idx], mode)
-    t_det = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    vals = c.project(start[idx], end[idx], mode)
-    dt = time.perf_counter() - t0
-    crossings = int(off[-1])
-    same = bool(np.array_equal(vals.reshape(gpu_vals[idx].shape), gpu_vals[idx]))
-    close = bool(np.allclose(vals.reshape(gpu_vals[idx].shape), gpu_vals[idx],
-                             rtol=1e-12, atol=0))
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"[bench] cpu reference: {len(idx)} rays, {crossings} crossings in "
-        f"{dt:.2f}s ({threads} threads); values bit-identical: {same}")
+    cpu = host_cpu_info()
+    share = int(os.environ.get("OMP_NUM_THREADS", cpu["affinity_cpus"]))
+    cores = cpu.get("physical_cores") or cpu["affinity_cpus"]
+    threads = max(1, min(share, cores, cpu["affinity_cpus"]))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    with tempfile.TemporaryDirectory() as tmp:
+        np.savez(os.path.join(tmp, "in.npz"), pos=pos, fields=fields, start=start[idx],
+                 end=end[idx])
+        cmd = [sys.executable, "-m", "oracle.cpu_time", os.path.join(tmp, "in.npz"),
+               os.path.join(tmp, "out.npz"), kind, mode, "1" if periodic else "0", "3"]
+        r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True)
+        if r.returncode != 0:
+            log(f"[bench] cpu baseline failed: {r.stderr[-2000:]}")
+            return None
+        res = np.load(os.path.join(tmp, "out.npz"))
+        vals, times, crossings = res["values"], res["seconds"], int(res["crossings"])
+    dt = float(np.median(times))
+    gv = gpu_vals[idx]
+    same = bool(np.array_equal(vals.reshape(gv.shape), gv))
+    close = bool(np.allclose(vals.reshape(gv.shape), gv, rtol=1e-12, atol=0))
+    log(f"[bench] cpu {kind}: {len(idx)} rays, {crossings} crossings, median "
+        f"{dt:.2f}s of {len(times)} ({threads} threads on {cpu['model']}); values "
+        f"bit-identical: {same}")
     return {"value": crossings / dt, "unit": "cell-crossings/s", "cores": threads,
-            "kind": "reference",
-            "sample": f"every {stride}th ray of the same image ({len(idx)} rays, "
-                      f"{crossings} crossings), reference C++ core "
-                      f"(makeProjection, OpenMP dynamic,256)",
+            "kind": kind,
+            "sample": f"every {stride}th ray of the same rays ({len(idx)} rays, "
+                      f"{crossings} crossings), "
+                      + ("the reference's C++ core built from its sources "
+                         "(makeProjection, OpenMP dynamic,256)" if kind == "reference" else
+                         "the oracle's C restatement (OpenMP dynamic,256)")
+                      + f"; median of {len(times)} runs",
+            "seconds": [float(t) for t in times],
             "rays_per_s": len(idx) / dt,
+            "cpu": cpu,
+            "omp": {"OMP_NUM_THREADS": threads, "OMP_PROC_BIND": "close",
+                    "OMP_PLACES": "cores"},
             "values_bit_identical_to_gpu": same,
             "values_within_1e-12": close}
 
@@ -353,16 +483,7 @@ def main():
     kind, mode = w.get("kind", "image"), w["mode"]
     pos, fields, views, params = make_workload(args.workload)
     if mode == "volume":
-        # kappa: median ray opacity 0.5 on a 64^2 z-probe of the box
-        pc_probe = vt.ProjectionCloud(pos, fields, boundbox=[0, 100, 0, 100, 0, 100],
-                                      periodic=True)
-        ps, pe = vt.grid.generate_projection_grid([0.0, 100.0], 64, [0.0, 100.0], None)
-        _, ids, se, sx, off = pc_probe.segments_tensor(
-            torch.from_numpy(ps.reshape(-1, 3)), torch.from_numpy(pe.reshape(-1, 3)))
-        kappa = synthetic.kappa_for_median_opacity(
-            fields, ids.cpu().numpy(), (sx - se).cpu().numpy(), off.cpu().numpy())
-        fields = synthetic.rgba_transfer(fields, kappa)
-        del pc_probe
+        fields, kappa = volume_fields(pos, fields)
         log(f"[bench] kappa = {kappa:.6e}")
 
     t0 = time.perf_counter()
@@ -459,9 +580,38 @@ def main():
         del grad, target
 
     bytes_alg = algorithmic_bytes(R, C, len(pos), n_f, n_out)
-    traffic, l2_hit = read_traffic(args.workload)
+    traffic = read_traffic(args.workload)
+    traffic_fa = read_traffic(args.workload + "_fwd_adjoint")
     achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
+    stream_gbs = stream_copy_gbs()
+    log(f"[bench] device copy: {stream_gbs:.0f} GB/s (read + write)")
+    if fa is not None:
+        # one walk: the recording forward's bytes plus the adjoint's; beside
+        # it the re-walk convention (forward twice, fields and image again)
+        b_one = bytes_alg + adjoint_bytes(R, C, len(pos), n_f, n_out, sort_passes(len(pos)))
+        b_rewalk = 2 * bytes_alg + C * 8 * n_f + R * 8 * n_out
+        k_ms = fa["forward_kernel_ms"] + fa["adjoint_kernel_ms"]
+        a_one = b_one / (k_ms * 1e-3) / 1e9
+        fa["roofline"] = {
+            "bound": "hbm", "kernel": "recording forward + adjoint kernels (one walk)",
+            "achieved": a_one, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": a_one / HBM_PEAK_GBS,
+            "traffic": traffic_fa["hbm_bytes_per_launch"] if traffic_fa else None,
+            "traffic_record": traffic_fa,
+            "algorithmic_bytes_per_step": b_one, "kernel_ms_per_step": k_ms,
+            "formula": "R(48 + 8 n_out) + C(8 n_f) + 2C(32 D + 240) [forward] + "
+                       "adjoint_bytes (bench.py) [records, sort, sums]",
+            "rewalk_algorithmic_bytes": b_rewalk,
+            "rewalk_frac": b_rewalk / (k_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "stream_copy_gbs": stream_gbs}
     step = fa if kind == "fit" else fwd
+    if kind == "views":
+        order_desc = "one launch over every view, index order"
+    else:
+        from vortrace_amd.autograd import _order_pays
+        est = _order_pays(R, pc._cloud.lanes(pc._validate_reduction(mode)))
+        order_desc = (f"8x8 tiles ({ray_row}-ray rows)" if ray_row else "index") + \
+            (", runs of 64 longest first by estimated cost" if est else "")
     result = {
         "metric": METRIC,
         "value": step["crossings_per_s"],
@@ -479,7 +629,7 @@ def main():
                    else "forward", "n_points": len(pos), "views": len(views), "rays": R,
                    "crossings_per_step": C, "periodic": w["periodic"],
                    "reduction": mode, "tree_depth": int(info["max_depth"]),
-                   "ray_order": f"8x8 tiles ({ray_row}-ray rows)" if ray_row else "index"},
+                   "ray_order": order_desc},
         "rays_per_s": step["rays_per_s"],
         "kernel_ms": kernel_ms,
         "kernel_crossings_per_s": C / (kernel_ms * 1e-3),
@@ -487,11 +637,15 @@ def main():
         "forward": fwd,
         "roofline": {"bound": "hbm", "kernel": "forward projection", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
-                     "hbm_gbs_measured": (traffic / (kernel_ms * 1e-3) / 1e9
-                                          if traffic else None),
-                     "l2_hit_rate": l2_hit,
-                     "algorithmic_bytes_per_launch": bytes_alg},
+                     "traffic": traffic["hbm_bytes_per_launch"] if traffic else None,
+                     "traffic_record": traffic,
+                     "hbm_gbs_at_profile": (
+                         traffic["hbm_bytes_per_launch"] / (traffic["profiled_kernel_ms"] * 1e-3)
+                         / 1e9 if traffic and traffic.get("profiled_kernel_ms") else None),
+                     "l2_hit_rate": traffic["l2_hit_rate"] if traffic else None,
+                     "algorithmic_bytes_per_launch": bytes_alg,
+                     "stream_copy_gbs": stream_gbs,
+                     "frac_of_stream_copy": achieved / stream_gbs},
         "fwd_adjoint": fa,
         "tuning": tuning,
         "setup_seconds": {"tree_build_host": info["build_seconds"],

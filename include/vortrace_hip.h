@@ -244,6 +244,14 @@ typedef struct {
 int vt_set_tuning(const vt_tuning *t);
 int vt_get_tuning(vt_tuning *t);
 
+/* Resident lanes of the forward kernel a vt_project_counted (recorded = 0)
+ * or vt_project_recorded (recorded != 0) call of `mode` on this cloud
+ * launches: its persistent grid times the workgroup size.  A launch with
+ * fewer rays leaves lanes idle; a recording launch leaves at most one partly
+ * filled chunk of 32 records per lane.  No reference counterpart (the
+ * reference's OpenMP loop has no fixed lane count, src/projection.cpp:43). */
+int vt_cloud_lanes(const vt_cloud *cloud, int mode, int recorded, int64_t *lanes);
+
 /* Per-call statistics of the last vt_project* call on this thread: total
  * segments (cell crossings), kNN queries run, the deepest walk and kNN
  * stacks, K_EXPAND searches, rays re-run by the K_EXPAND-capable kernel, and

@@ -235,6 +235,8 @@ def test_autograd_records_after_first_pass():
         f = f0.clone().requires_grad_(True)
         (vt.autograd.project(pc, f, ds_, de_, "sum") * w).sum().backward()
         grads.append(f.grad)
-        assert pc._segment_estimate > 0
+        states = list(vt.autograd._ray_states(pc).values())
+        assert len(states) == 1 and states[0].estimate > 0
+        assert pc._adjoint_path == ("walk" if len(grads) == 1 else "recorded")
     assert torch.equal(grads[0], grads[1]) and torch.equal(grads[1], grads[2])
     assert int(torch.count_nonzero(grads[0])) > 0

@@ -222,7 +222,7 @@ def test_longest_first_gradient_same_bits(clouds, monkeypatch):
     (here at any ray count): the image and gradient of every pass equal the
     first (index-order) pass."""
     import vortrace_amd.autograd as ag
-    monkeypatch.setattr(ag, "_order_pays", lambda n, device: True)
+    monkeypatch.setattr(ag, "_order_pays", lambda n, lanes: True)
     _say("longest-first gradient passes")
     pos, rho = synthetic.clustered_mesh(20_000, seed=85, n_clumps=8)
     pc = vt.ProjectionCloud(pos, rho, boundbox=BB, periodic=True)
@@ -236,7 +236,8 @@ def test_longest_first_gradient_same_bits(clouds, monkeypatch):
         img = pc.projection_grad(ds_, de_, x, reduction="integrate")
         (img * g).sum().backward()
         res.append((img.detach().clone(), x.grad.clone()))
-    assert pc._longest_first is not None
+    states = list(ag._ray_states(pc).values())
+    assert len(states) == 1 and states[0].order is not None
     for img, gr in res[1:]:
         assert torch.equal(img, res[0][0]) and torch.equal(gr, res[0][1])
 
@@ -249,13 +250,13 @@ def test_estimated_order_same_bits(periodic):
     pos, rho = synthetic.clustered_mesh(20_000, seed=91, n_clumps=6)
     pc = vt.ProjectionCloud(pos, rho, boundbox=BB, periodic=periodic)
     dev = torch.device("cuda")
-    lanes = 512 * torch.cuda.get_device_properties(dev).multi_processor_count
+    lanes = pc._cloud.lanes(vt.ReductionMode.Sum)
     npix = int(np.ceil(np.sqrt(1.5 * lanes)))
     s, e = grid.generate_projection_grid([0.0, 100.0], npix, [0.0, 100.0], None)
     ds_ = torch.from_numpy(s.reshape(-1, 3)).to(dev)
     de_ = torch.from_numpy(e.reshape(-1, 3)).to(dev)
     ref = vt.Projection(ds_, de_).project_tensor(pc._cloud, vt.ReductionMode.Sum)
-    assert pc._estimated_order(vt.Projection(ds_, de_)) is not None
+    assert pc._estimated_order(vt.Projection(ds_, de_), vt.ReductionMode.Sum) is not None
     pc.set_ray_rows(npix)
     try:
         got = pc.projection_tensor(ds_, de_)
@@ -271,7 +272,7 @@ def test_grid_projection_estimated_order():
     pos, rho = synthetic.clustered_mesh(20_000, seed=92, n_clumps=6)
     pc = vt.ProjectionCloud(pos, rho, boundbox=BB, periodic=True)
     dev = torch.device("cuda")
-    lanes = 512 * torch.cuda.get_device_properties(dev).multi_processor_count
+    lanes = pc._cloud.lanes(vt.ReductionMode.Sum)
     npix = int(np.ceil(np.sqrt(2.5 * lanes)))
     _say(f"grid_projection {npix}^2 with the estimated order")
     img = pc.grid_projection([0.0, 100.0], npix, [0.0, 100.0], None)

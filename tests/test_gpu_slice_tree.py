@@ -68,3 +68,56 @@ def test_projection_through_loaded_tree():
         q.load_tree_bytes(data)
         np.testing.assert_array_equal(q.projection(s, e, reduction="sum"), expect)
         assert q.save_tree_bytes() == dumps[0]
+
+
+def test_tree_file_and_ray_class(tmp_path):
+    """PointCloud.saveTree / loadTree with files (src/pointcloud.cpp:214-240)
+    and the Ray class (src/bindings.cpp:277-307) against the reference's
+    build (the oracle where it is absent)."""
+    _say("tree files and Ray")
+    pos, rho = synthetic.uniform_mesh(20000, seed=23)
+    f4 = np.column_stack([0.2 + 0.5 * np.minimum(rho / 10, 1), np.full(len(rho), 0.3),
+                          np.full(len(rho), 0.6), 0.02 * rho])
+    bb = [0, 100, 0, 100, 0, 100]
+    cpu = (ref.RefCloud if ref.available() else port.OracleCloud)(pos, f4, bb, periodic=True)
+    pc = vt.PointCloud()
+    pc.loadPoints(pos, f4, bb, periodic=True)
+    with pytest.raises(RuntimeError, match="Cannot save tree: tree has not been built"):
+        pc.saveTree(str(tmp_path / "t.bin"))
+    pc.buildTree()
+    path = tmp_path / "tree.bin"
+    pc.saveTree(str(path))
+    if ref.available():
+        assert path.read_bytes() == cpu.tree_bytes()
+    with pytest.raises(RuntimeError, match="Cannot open file for writing"):
+        pc.saveTree(str(tmp_path / "missing" / "t.bin"))
+    q = vt.PointCloud()
+    q.loadPoints(pos, f4, bb, periodic=True)
+    with pytest.raises(RuntimeError, match="Cannot open file for reading"):
+        q.loadTree(str(tmp_path / "nothing.bin"))
+    q.loadTree(str(path))
+    assert q.saveTreeToBytes() == path.read_bytes()
+    empty = vt.PointCloud()
+    empty.loadPoints(np.zeros((0, 3)), np.zeros(0), bb, periodic=True)
+    with pytest.raises(RuntimeError, match="Cannot load tree: no points loaded"):
+        empty.loadTree(str(path))
+    start, end = np.array([3.0, 97.0, 0.5]), np.array([88.0, 12.0, 99.5])
+    for mode, name, get in ((vt.ReductionMode.Sum, "sum", "get_col"),
+                            (vt.ReductionMode.Max, "max", "get_max_val"),
+                            (vt.ReductionMode.Min, "min", "get_min_val"),
+                            (vt.ReductionMode.VolumeRender, "volume", "get_vol_render_val")):
+        r = vt.Ray(start, end)
+        assert getattr(r, get)() == [] and r.get_dens_col() == 0.0
+        r.integrate(q, mode)
+        vals, ids, se, sx, off = cpu.detailed(start[None], end[None], name)
+        if name == "volume":
+            np.testing.assert_allclose(getattr(r, get)(), vals[0], rtol=1e-12, atol=0)
+        else:
+            assert getattr(r, get)() == [float(x) for x in vals[0]]
+        segs = r.get_segments()
+        assert [s[0] for s in segs] == ids.tolist()
+        assert [s[1] for s in segs] == se.tolist() and [s[2] for s in segs] == sx.tolist()
+        assert [s[3] for s in segs] == (sx - se).tolist()
+        if name == "sum":
+            assert r.get_dens_col() == vals[0][0]
+    assert vt.Ray(start, end).walk(q) == segs

@@ -16,7 +16,8 @@ from . import _lib
 from . import autograd
 from . import grid
 from . import tuning
-from .core import PointCloud, Projection, ReductionMode, Slice, get_verbose, set_verbose
+from .core import (PointCloud, Projection, Ray, ReductionMode, Slice, get_verbose,
+                   set_verbose)
 from .vortrace import ProjectionCloud
 
 _lib.lib()  # fail loudly at import if the native library is missing
@@ -24,5 +25,5 @@ _lib.lib()  # fail loudly at import if the native library is missing
 logging.getLogger("vortrace").addHandler(logging.NullHandler())
 
 __version__ = "0.1.0"
-__all__ = ["ProjectionCloud", "PointCloud", "Projection", "ReductionMode", "Slice",
+__all__ = ["ProjectionCloud", "PointCloud", "Projection", "Ray", "ReductionMode", "Slice",
            "autograd", "grid", "tuning", "set_verbose", "get_verbose", "__version__"]

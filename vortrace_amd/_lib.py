@@ -82,6 +82,7 @@ SIGNATURES = {
     "vt_set_tuning": (I, [C.POINTER(Tuning)]),
     "vt_get_tuning": (I, [C.POINTER(Tuning)]),
     "vt_last_call_stats": (I, [C.POINTER(CallStats)]),
+    "vt_cloud_lanes": (I, [P, I, I, C.POINTER(C.c_int64)]),
 }
 
 _lib = None

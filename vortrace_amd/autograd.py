@@ -14,15 +14,17 @@ The rays are walked once per forward/backward pair: the forward pass records
 every segment (its cell, ray, place along the ray and length) in a device
 pool while it projects, and the backward pass reorders those records instead
 of walking the rays again (``vt_project_recorded`` / ``vt_adjoint_recorded``).
-The pool is sized from the segment count of the previous forward pass on the
-same cloud; on the first pass, or when the pool overflows, the backward pass
-walks the rays for their segments (``vt_adjoint``).  Both give the same bits.
+The pool is sized from the segment count of the previous forward pass over
+the same rays (the same ray tensors or arrays, unchanged; a few ray sets are
+remembered per cloud); on the first pass over a ray set, or when the pool
+overflows, the backward pass walks the rays for their segments
+(``vt_adjoint``).  Both give the same bits.
 
-From the second pass on, when the rays are one to eight times the lanes the
-kernel keeps resident, the forward pass also takes them longest first (by the
-previous pass's segment counts, in whole runs of 64 neighbouring rays within
-blocks of each ray partition), which shortens the kernel's tail; the order
-never changes a result.
+From the second pass over a ray set on, when the rays are one to eight times
+the lanes the kernel keeps resident, the forward pass also takes them longest
+first (by the previous pass's segment counts for the same rays, in whole runs
+of 64 neighbouring rays within blocks of each ray partition), which shortens
+the kernel's tail; the order never changes a result.
 
 The reference has no derivatives; the tree and the segments are the
 reference's, so the gradient is the exact derivative of its projection.
@@ -31,22 +33,25 @@ src/pointcloud.cpp:55-108) get a zero gradient.
 """
 from __future__ import annotations
 
+import collections
 import functools
+import warnings
 
+import numpy as np
 import torch
 
 from . import _lib
 from .core import SEGREC_BYTES, Projection, ReductionMode
 
-# Pool headroom over the previous pass's segment count: a fraction, and whole
-# record chunks left partly filled by the kernel's lanes (at most one chunk of
-# 32 records per lane of each of the two launches).
+# Pool headroom over the previous pass's segment count: a fraction, and the
+# records the kernels' lanes can leave unused (at most one partly filled chunk
+# of 32 records per resident lane, in each of the two launches of a call).
 _POOL_GROWTH = 1.05
-_POOL_SLACK = 2 * 131072 * 32
+_REC_CHUNK = 32
 
 
-def _pool_capacity(estimate: int) -> int:
-    return int(estimate * _POOL_GROWTH) + _POOL_SLACK
+def _pool_capacity(estimate: int, lanes: int) -> int:
+    return int(estimate * _POOL_GROWTH) + 2 * lanes * _REC_CHUNK
 
 
 # Rays per block of the longest-first order: 2^15 rays of an image are a band
@@ -54,15 +59,46 @@ def _pool_capacity(estimate: int) -> int:
 _ORDER_BLOCK = 1 << 15
 
 
-def _order_pays(n: int, device) -> bool:
+def _order_pays(n: int, lanes: int) -> bool:
     """Whether the longest-first order is worth its loss of ray locality: only
-    when each resident lane takes more than one ray and at most eight (about
-    512 lanes per CU are resident).  Measured on the forward kernel: 2 rays
-    per lane (512^2 rays, 2M clustered cells) -9 %; 8 rays per lane (1024^2,
-    volume) -2.5 %; half a ray per lane (256^2) +17 %; 32 rays per lane (16
-    views of 512^2) +1.5 %."""
-    lanes = 512 * torch.cuda.get_device_properties(device).multi_processor_count
+    when each of the kernel's `lanes` resident lanes takes more than one ray
+    and at most eight.  Measured on the forward kernel: 2 rays per lane (512^2
+    rays, 2M clustered cells) -9 %; 8 rays per lane (1024^2, volume) -2.5 %;
+    half a ray per lane (256^2) +17 %; 32 rays per lane (16 views of 512^2)
+    +1.5 %."""
     return lanes < n <= 8 * lanes
+
+
+def _rays_key(pos_start, pos_end):
+    """Identity of a ray set between passes: tensors by storage, shape,
+    version counter and device; arrays by address and shape.  Used only to
+    size the record pool and order the rays (speed, never results)."""
+    def one(x):
+        if isinstance(x, torch.Tensor):
+            return ("t", x.data_ptr(), tuple(x.shape), x._version, str(x.device), str(x.dtype))
+        a = np.asarray(x)
+        return ("n", a.__array_interface__["data"][0], a.shape, a.dtype.str)
+    return one(pos_start), one(pos_end)
+
+
+class _RayState:
+    """What a forward pass leaves for the next pass over the same rays."""
+    __slots__ = ("estimate", "order")
+
+    def __init__(self, estimate, order):
+        self.estimate = estimate
+        self.order = order
+
+
+_MAX_RAY_STATES = 8
+
+
+def _ray_states(pc) -> collections.OrderedDict:
+    st = getattr(pc, "_ray_state_lru", None)
+    if st is None:
+        st = collections.OrderedDict()
+        pc._ray_state_lru = st
+    return st
 
 
 def tiled_slots(n: int, row: int, device=None) -> torch.Tensor:
@@ -129,8 +165,10 @@ def longest_first(counts: torch.Tensor, row: int = 0,
     run_max = torch.zeros(run_len.numel(), dtype=torch.int64, device=dev)
     run_max.scatter_reduce_(0, run, counts.to(torch.int64)[rays], reduce="amax",
                             include_self=False)
-    # Sort the runs (not the rays) and lay them out again in that order.
-    span = 1 << 40  # above any segment count
+    # Sort the runs (not the rays) and lay them out again in that order; a
+    # run's largest count is clamped below the span so groups stay apart.
+    span = 1 << 40
+    run_max.clamp_(0, span - 1)
     perm = torch.argsort(group * span + (span - 1 - run_max), stable=True)
     plen = run_len[perm]
     out_start = torch.cumsum(plen, 0) - plen
@@ -146,35 +184,50 @@ class ProjectFields(torch.autograd.Function):
     def forward(ctx, fields, pc, pos_start, pos_end, mode):
         pc.set_fields(fields)
         proj = Projection(pos_start, pos_end)
-        dev = pc._cloud._device
+        cloud = pc._cloud
+        dev = cloud._device
+        key = _rays_key(pos_start, pos_end)
+        states = _ray_states(pc)
+        state = states.get(key)
+        if state is not None:
+            states.move_to_end(key)
         counts = torch.empty(proj.ngrid, dtype=torch.int64, device=dev)
         records, n_records = None, -1
-        estimate = getattr(pc, "_segment_estimate", None)
-        # The previous pass's counts order this pass's rays longest first
-        # (speed only; any permutation gives the same bits).
-        order = getattr(pc, "_longest_first", None)
+        # The previous pass's counts over the same rays order this pass's
+        # rays longest first (speed only; any permutation gives the same bits).
+        order = state.order if state is not None else None
         ordered = (order is not None and order.numel() == proj.ngrid
-                   and not pc._cloud.get_ray_order_size())
+                   and not cloud.get_ray_order_size())
         if ordered:
-            pc._cloud.set_ray_order(order)
+            cloud.set_ray_order(order)
         try:
-            if ctx.needs_input_grad[0] and estimate is not None:
-                records = torch.empty((_pool_capacity(estimate), SEGREC_BYTES),
+            if ctx.needs_input_grad[0] and state is not None:
+                lanes = cloud.lanes(mode, recorded=True)
+                records = torch.empty((_pool_capacity(state.estimate, lanes), SEGREC_BYTES),
                                       dtype=torch.uint8, device=dev)
-                out, n_records = proj.project_recorded_tensor(pc._cloud, mode, counts,
-                                                              records)
+                out, n_records = proj.project_recorded_tensor(cloud, mode, counts, records)
                 if n_records < 0:
                     records = None  # pool too small: the backward walks the rays
+                    if not getattr(pc, "_pool_warned", False):
+                        pc._pool_warned = True
+                        warnings.warn("vortrace_amd.autograd: the segment record pool "
+                                      "overflowed; the backward pass walks the rays again "
+                                      "(same result, slower)", RuntimeWarning, stacklevel=2)
             else:
-                out = proj.project_tensor(pc._cloud, mode, counts=counts)
+                out = proj.project_tensor(cloud, mode, counts=counts)
             pc._forward_stats = _lib.last_call_stats()
         finally:
             if ordered:
-                pc._cloud.set_ray_order(None)
+                cloud.set_ray_order(None)
         if ctx.needs_input_grad[0]:
-            pc._segment_estimate = int(counts.sum().item())
-            pc._longest_first = (longest_first(counts, pc._cloud.get_ray_rows())
-                                 if _order_pays(counts.numel(), dev) else None)
+            lanes = cloud.lanes(mode, recorded=True)
+            states[key] = _RayState(
+                int(counts.sum().item()),
+                longest_first(counts, cloud.get_ray_rows())
+                if _order_pays(counts.numel(), lanes) else None)
+            states.move_to_end(key)
+            while len(states) > _MAX_RAY_STATES:
+                states.popitem(last=False)
         ctx.save_for_backward(fields, counts)
         ctx.records, ctx.n_records = records, n_records
         ctx.pc, ctx.proj, ctx.mode = pc, proj, mode
@@ -190,8 +243,10 @@ class ProjectFields(torch.autograd.Function):
         if ctx.records is not None:
             gf = ctx.proj.adjoint_recorded_tensor(pc._cloud, g, mode, counts, ctx.records,
                                                   ctx.n_records)
+            pc._adjoint_path = "recorded"
         else:
             gf = ctx.proj.adjoint_tensor(pc._cloud, g, mode, counts=counts)
+            pc._adjoint_path = "walk"
         # Statistics are per thread, and autograd may run this on its own.
         pc._adjoint_stats = _lib.last_call_stats()
         ctx.records = None

@@ -216,6 +216,31 @@ class PointCloud:
         _lib.check(_lib.lib().vt_cloud_load_tree(self._handle(), buf, len(data), 1))
         self._device = torch.device("cuda", torch.cuda.current_device())
 
+    def saveTree(self, filename: str) -> None:
+        """PointCloud::saveTree (src/pointcloud.cpp:214-225): the tree as
+        nanoflann's index dump in a file."""
+        data = self.saveTreeToBytes()
+        try:
+            f = open(filename, "wb")
+        except OSError:
+            raise RuntimeError(f"Cannot open file for writing: {filename}") from None
+        with f:
+            f.write(data)
+
+    def loadTree(self, filename: str) -> None:
+        """PointCloud::loadTree (src/pointcloud.cpp:227-240): replaces the
+        tree with the index dump in a file and uploads it to the current
+        device."""
+        if self._h is None or self.get_npart() == 0:
+            raise RuntimeError("Cannot load tree: no points loaded")
+        try:
+            f = open(filename, "rb")
+        except OSError:
+            raise RuntimeError(f"Cannot open file for reading: {filename}") from None
+        with f:
+            data = f.read()
+        self.loadTreeFromBytes(data)
+
     def load_host_tree_bytes(self, data: bytes) -> None:
         """loadTreeFromBytes without the upload (no device needed)."""
         data = bytes(data)
@@ -268,6 +293,15 @@ class PointCloud:
         _lib.check(_lib.lib().vt_cloud_set_fields(self._handle(), _P(f.data_ptr()),
                                                   _stream()))
         self._fields_dev = f.reshape(self.get_npart(), nf)
+
+    def lanes(self, reduction=Sum, recorded: bool = False) -> int:
+        """Resident lanes of the forward kernel a projection with `reduction`
+        (recording its segments when `recorded`) launches on this cloud
+        (vt_cloud_lanes)."""
+        n = C.c_int64()
+        _lib.check(_lib.lib().vt_cloud_lanes(self._handle(), int(reduction),
+                                             int(bool(recorded)), C.byref(n)))
+        return int(n.value)
 
     # -- queries -----------------------------------------------------------------
     def query_tensor(self, q: torch.Tensor, k: int):
@@ -457,6 +491,55 @@ class Projection:
         if self._segments is None:
             raise RuntimeError("No detailed projection data available")
         return tuple(t.cpu().numpy() for t in self._segments)
+
+
+class Ray:
+    """One ray (Ray, include/ray.hpp; binding src/bindings.cpp:277-307): the
+    walk through a cloud and its reductions, on the GPU.  Segments are
+    (cell id, s_enter, s_exit, ds) with cell ids as filtered indices, as the
+    reference's get_segments gives them."""
+
+    def __init__(self, start, end):
+        self._start = np.ascontiguousarray(start, dtype=np.float64).reshape(1, 3)
+        self._end = np.ascontiguousarray(end, dtype=np.float64).reshape(1, 3)
+        self._segments = []
+        self._vals = {}
+
+    def _trace(self, cloud: PointCloud, mode):
+        v, ids, se, sx, _ = Projection(self._start, self._end).segments_tensor(cloud, mode)
+        ids, se, sx = ids.cpu().numpy(), se.cpu().numpy(), sx.cpu().numpy()
+        self._segments = [(int(i), float(a), float(b), float(b - a))
+                          for i, a, b in zip(ids, se, sx)]
+        return [float(x) for x in v.cpu().numpy()[0]]
+
+    def walk(self, cloud: PointCloud):
+        """Ray::walk (src/ray.cpp:187-430): traces the ray; returns its
+        segments (the reference's binding cannot convert its return value)."""
+        self._trace(cloud, Sum)
+        return self.get_segments()
+
+    def integrate(self, cloud: PointCloud, mode=Sum) -> None:
+        """Ray::integrate (src/ray.cpp:432-442): walk, then reduce by `mode`."""
+        self._vals[int(mode)] = self._trace(cloud, ReductionMode(int(mode)))
+
+    def get_dens_col(self) -> float:
+        col = self._vals.get(int(Sum), [])
+        return col[0] if col else 0.0
+
+    def get_col(self):
+        return list(self._vals.get(int(Sum), []))
+
+    def get_max_val(self):
+        return list(self._vals.get(int(Max), []))
+
+    def get_min_val(self):
+        return list(self._vals.get(int(Min), []))
+
+    def get_vol_render_val(self):
+        return list(self._vals.get(int(VolumeRender), []))
+
+    def get_segments(self):
+        return list(self._segments)
 
 
 class Slice:

@@ -40,7 +40,8 @@ enum : int {
   kErrPeriodicStart = 6,   // src/ray.cpp:243-246
   kErrWalkStack = 7,       // walk stack deeper than the allocated capacity
   kErrKnnStack = 8,        // kNN stack deeper than the tree (cannot happen)
-  kErrTies = 9,            // >= K_EXPAND equidistant generators at one point
+  kErrTies = 9,            // >= K_EXPAND equidistant generators at one point,
+                           // periodic images merged
   kErrNonFinite = 10,      // a query found no generator (non-finite point)
   kErrFillOverflow = 11,   // fill pass produced more segments than counted
   // Not an error: the ray needs the K_EXPAND classification, which the fast

@@ -567,12 +567,15 @@ __device__ void query_reg(const DevCloud &C, const double q0, const double q1, c
 
 // queryTreeK for a runtime k (<= kMaxQueryK) with results in `res`
 // (k entries) and `merged` workspace (2k entries), both in device memory.
-// Entries beyond the number of points are (kNone, DBL_MAX).
+// Entries beyond the number of points are (kNone, DBL_MAX).  Returns true when
+// at least one periodic image was searched and merged (the merge then sorted
+// k or more entries; see the note at the sort).
 template <bool kPeriodic, class ST>
-__device__ void query_mem(const DevCloud &C, const double q0, const double q1, const double q2,
+__device__ bool query_mem(const DevCloud &C, const double q0, const double q1, const double q2,
                           int k, MemRes &res, uint32_t *mi, double *md, ST &st) {
   knn_core(C, q0, q1, q2, res, st);
-  if (!kPeriodic) return;
+  if (!kPeriodic) return false;
+  bool merged = false;
   int m = k;
   for (int j = 0; j < k; ++j) {
     gst(mi + j, gld(res.id + j));
@@ -590,6 +593,7 @@ __device__ void query_mem(const DevCloud &C, const double q0, const double q1, c
         const double sh[3] = {qv[0] + ix * C.box[0], qv[1] + iy * C.box[1],
                               qv[2] + iz * C.box[2]};
         knn_core(C, sh[0], sh[1], sh[2], img, st);
+        merged = true;
         for (int i = 0; i < k; ++i) {
           const uint32_t iid = gld(img.id + i);
           const double idd = gld(img.d + i);
@@ -609,8 +613,8 @@ __device__ void query_mem(const DevCloud &C, const double q0, const double q1, c
           }
         }
         // Stable insertion sort.  libstdc++'s std::sort is an insertion sort
-        // (stable) up to 16 elements; beyond that its order among equal r^2
-        // may differ, which only the k > 8 query hook can observe.
+        // (stable) up to 16 elements; beyond that (k > 8) it is an introsort
+        // whose order among equal r^2 may differ from this one.
         for (int i = 1; i < m; ++i) {
           const double vd = gld(md + i);
           const uint32_t vi = gld(mi + i);
@@ -629,6 +633,7 @@ __device__ void query_mem(const DevCloud &C, const double q0, const double q1, c
     gst(res.id + j, gld(mi + j));
     gst(res.d + j, gld(md + j));
   }
+  return merged;
 }
 
 }  // namespace vt

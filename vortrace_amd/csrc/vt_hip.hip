@@ -765,7 +765,8 @@ int finish_call(vt_cloud *c, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool 
                                    (long long)er.ray));
     case kErrTies:
       return fail(VT_ERUNTIME, fmt("Ray %lld: 100 or more generators equidistant from one "
-                                   "split point; classification is ill-defined",
+                                   "split point across periodic images; classification is "
+                                   "ill-defined",
                                    (long long)er.ray));
     default:
       return fail(VT_ERUNTIME, fmt("Ray %lld: kNN traversal stack overflow", (long long)er.ray));
@@ -869,6 +870,17 @@ int project(vt_cloud *c, const double *s, const double *e, int64_t n, double *ou
             int64_t *counts, hipStream_t st, const RecPool &pool = RecPool{}) {
   return c->periodic ? project_p<MODE, NF, true, REC>(c, s, e, n, out, nout, counts, pool, st)
                      : project_p<MODE, NF, false, REC>(c, s, e, n, out, nout, counts, pool, st);
+}
+
+// Resident lanes (persistent grid x kBlock) of the forward kernel that a
+// projection call of `mode` on cloud c launches (the recording variant for
+// REC); every launch of that kernel runs this many lanes.
+template <int MODE, int NF, bool REC>
+int64_t project_lanes(const vt_cloud *c) {
+  const int grid = c->periodic
+                       ? grid_for(k_project<MODE, NF, true, REC>, INT64_MAX / 2, c->device)
+                       : grid_for(k_project<MODE, NF, false, REC>, INT64_MAX / 2, c->device);
+  return (int64_t)grid * kBlock;
 }
 
 int check_mode(const vt_cloud *c, int mode) {
@@ -1818,6 +1830,36 @@ int vt_get_tuning(vt_tuning *t) {
   t->leaf_batch = leaf_batch();
   t->ray_parts = ray_parts();
   return VT_OK;
+}
+
+int vt_cloud_lanes(const vt_cloud *cc, int mode, int recorded, int64_t *lanes) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (!rc) rc = check_mode(cc, mode);
+    if (rc) return rc;
+    if (!lanes) return fail(VT_EINVAL, "null argument");
+    if (recorded && mode != VT_SUM && mode != VT_VOLUME)
+      return fail(VT_EINVAL, "recorded projections are for reductions 'sum' and 'volume'");
+    const size_t nf = cc->nf;
+    if (recorded) {
+      *lanes = mode == VT_VOLUME ? project_lanes<VT_VOLUME, 4, true>(cc)
+               : nf == 1         ? project_lanes<VT_SUM, 1, true>(cc)
+                                 : project_lanes<VT_SUM, 0, true>(cc);
+    } else if (mode == VT_SUM) {
+      *lanes = nf == 1   ? project_lanes<VT_SUM, 1, false>(cc)
+               : nf == 2 ? project_lanes<VT_SUM, 2, false>(cc)
+               : nf == 3 ? project_lanes<VT_SUM, 3, false>(cc)
+               : nf == 4 ? project_lanes<VT_SUM, 4, false>(cc)
+                         : project_lanes<VT_SUM, 0, false>(cc);
+    } else if (mode == VT_MAX) {
+      *lanes = project_lanes<VT_MAX, 0, false>(cc);
+    } else if (mode == VT_MIN) {
+      *lanes = project_lanes<VT_MIN, 0, false>(cc);
+    } else {
+      *lanes = project_lanes<VT_VOLUME, 4, false>(cc);
+    }
+    return VT_OK;
+  });
 }
 
 int vt_last_call_stats(vt_call_stats *out) {

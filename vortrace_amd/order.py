@@ -40,8 +40,8 @@ def density_grid(pc, device) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
 
 
 def ray_costs(pc, start: torch.Tensor, end: torch.Tensor) -> torch.Tensor:
-    """Estimated cost of each ray (float32): its length times the sum of
-    (points per cell)^(1/3) at S points along it."""
+    """Estimated cost of each ray (float32): its length (in box diagonals)
+    times the sum of (points per cell)^(1/3) at S points along it."""
     grid, lo, size = density_grid(pc, start.device)
     s = start.float()
     d = end.float() - s
@@ -50,7 +50,10 @@ def ray_costs(pc, start: torch.Tensor, end: torch.Tensor) -> torch.Tensor:
     ijk = torch.floor((p - lo) / size * _G).long()
     ijk = torch.remainder(ijk, _G) if pc.periodic else ijk.clamp_(0, _G - 1)
     flat = (ijk[..., 0] * _G + ijk[..., 1]) * _G + ijk[..., 2]
-    return grid[flat].sum(dim=1) * torch.linalg.vector_norm(d, dim=1)
+    # length in units of the box diagonal: the cost does not scale with the
+    # caller's coordinate units
+    return grid[flat].sum(dim=1) * (torch.linalg.vector_norm(d, dim=1)
+                                    / torch.linalg.vector_norm(size))
 
 
 def estimated_order(pc, start: torch.Tensor, end: torch.Tensor, row: int = 0):

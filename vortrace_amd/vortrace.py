@@ -105,7 +105,7 @@ class ProjectionCloud:
         p = Projection(start.reshape(-1, 3), end.reshape(-1, 3))
         self._require_tree()
         with self._image_rows(shape[1]):
-            order = self._estimated_order(p)
+            order = self._estimated_order(p, mode)
             if order is None:
                 p.makeProjection(self._cloud, mode)
             else:
@@ -221,7 +221,7 @@ class ProjectionCloud:
         mode = self._validate_reduction(reduction)
         self._require_tree()
         proj = Projection(pos_start, pos_end)
-        order = self._estimated_order(proj)
+        order = self._estimated_order(proj, mode)
         if order is None:
             return proj.project_tensor(self._cloud, mode)
         self._cloud.set_ray_order(order)
@@ -230,13 +230,14 @@ class ProjectionCloud:
         finally:
             self._cloud.set_ray_order(None)
 
-    def _estimated_order(self, proj):
+    def _estimated_order(self, proj, mode=ReductionMode.Sum):
         """Longest-first order of the rays by estimated cost (order.py) when it
-        pays (autograd._order_pays) and the caller set no order of its own."""
+        pays (autograd._order_pays, with the lanes of the kernel the call
+        launches) and the caller set no order of its own."""
         from .autograd import _order_pays
         from .order import estimated_order
-        dev = self._cloud._device
-        if self._cloud.get_ray_order_size() or not _order_pays(proj.ngrid, dev):
+        if self._cloud.get_ray_order_size() or not _order_pays(proj.ngrid,
+                                                               self._cloud.lanes(mode)):
             return None
         s, e = proj._rays(self._cloud)
         return estimated_order(self, s, e, self._cloud.get_ray_rows())
