@@ -166,6 +166,26 @@ int vt_cloud_set_ray_rows(vt_cloud *cloud, int64_t row);
  * takes rays in index order). */
 int vt_cloud_set_ray_order(vt_cloud *cloud, const uint32_t *d_order, int64_t n, void *stream);
 
+/* Sets the cloud's ray order for its coming calls of exactly n rays (as
+ * vt_cloud_set_ray_order) to longest first, built on the device: fetch slot j
+ * names ray j, or with row > 0 the 8x8-tile order of row-major images of
+ * `row` rays per row; the slots of each ray partition (vt_tuning.ray_parts
+ * contiguous ranges) are cut into blocks of `block` slots (1..32768) and
+ * runs of 64, and each block takes its runs by descending key (lower run
+ * first among equals).  A run's key is the largest of d_counts (n segment
+ * counts of a previous pass over the same rays, device) over its rays, or,
+ * when d_counts is NULL, an estimate from the run's middle ray (d_start /
+ * d_end, n x 3): its length in box diagonals times the sum of (points per
+ * cell)^(1/3) of a 32^3 histogram of the generators over the cloud's box at
+ * 32 points along it, x 1024, truncated.  d_order_out (n, optional) receives
+ * the order; d_run_keys (optional) the keys of the parts *
+ * ceil(ceil(n / parts) / block) * ceil(block / 64) runs.  Speed only.
+ * No reference counterpart (src/projection.cpp:43-68 takes rays in index
+ * order). */
+int vt_cloud_order_rays(vt_cloud *cloud, const double *d_start, const double *d_end,
+                        const int64_t *d_counts, int64_t n, int64_t row, int64_t block,
+                        uint32_t *d_order_out, int64_t *d_run_keys, void *stream);
+
 /* Replaces the cloud's fields, given on the device in filtered point order
  * (row i belongs to point vt_cloud_orig_ids()[i]; n x nfields, the nfields of
  * vt_cloud_create), without rebuilding the tree, and re-applies the RGBA

@@ -122,7 +122,7 @@ def _sphere_cloud(seed, r=45, keep=118):
     return p, 1.0 + 0.01 * np.arange(len(p))
 
 
-@pytest.mark.parametrize("seed", [0, 3, 7, 12])
+@pytest.mark.parametrize("seed", [0, 3, 7])
 def test_expand_all_equidistant_non_periodic(seed):
     r = 45
     pos, f = _sphere_cloud(seed, r)

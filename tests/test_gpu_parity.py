@@ -281,3 +281,24 @@ def test_bounded_searches_across_scales(scale, periodic):
     s, e = grid.generate_projection_grid([0.0, scale], 12, [0.0, scale], None)
     off = _cmp_segments(pc, oc, s.reshape(-1, 3), e.reshape(-1, 3))
     assert off[-1] > 2000
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_zero_length_ray_among_others(periodic):
+    """A zero-length ray (start == end) in a call with many ordinary rays: one
+    segment of length 0 in its start cell (src/ray.cpp:212-218) without
+    periodic images; with them the reference's dyadic probe of a zero-length
+    ray divides by zero, so only the non-periodic case is compared."""
+    pos, rho = synthetic.clustered_mesh(20_000, seed=93, n_clumps=6)
+    bb = [0, 100, 0, 100, 0, 100]
+    s, e = grid.generate_projection_grid([0, 100], 40, [0, 100], None)
+    s, e = s.reshape(-1, 3).copy(), e.reshape(-1, 3).copy()
+    e[9] = s[9]
+    e[700] = s[700] = (37.5, 61.25, 48.0)
+    pc = vt.ProjectionCloud(pos, rho, boundbox=bb, periodic=periodic)
+    if periodic:
+        keep = np.ones(len(s), bool)
+        keep[[9, 700]] = False
+        s, e = s[keep], e[keep]
+    oc = port.OracleCloud(pos, rho, bb, periodic=periodic)
+    _cmp_segments(pc, oc, s, e)

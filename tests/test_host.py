@@ -292,37 +292,3 @@ def test_longest_first_order(n, row, parts, block):
             expect = [fetch[sl] for k in ranked for sl in runs[k]]
             assert list(o[lo:hi]) == expect
 
-
-def test_estimated_order_is_a_permutation_led_by_dense_rays():
-    """order.estimated_order (on CPU tensors): a permutation of the rays, also
-    for degenerate and out-of-box rays; the run of 64 rays through the
-    densest cells goes first in its ray partition."""
-    import torch
-    from vortrace_amd import tuning
-    from vortrace_amd.order import estimated_order
-
-    class _Cloud:  # the attributes estimated_order reads
-        periodic = True
-        boundbox = [0, 100, 0, 100, 0, 100]
-
-    rng = np.random.default_rng(5)
-    pts = np.concatenate([rng.uniform(0, 100, (5000, 3)),
-                          rng.normal((20, 70, 50), 1.0, (5000, 3)) % 100])
-    pc = _Cloud()
-    pc.pos_orig = pts
-    pc.orig_ids = np.arange(len(pts))
-    n = 2048  # 8 partitions of 4 runs
-    s = np.column_stack([rng.uniform(40, 100, n), rng.uniform(0, 50, n), np.zeros(n)])
-    s[192:256] = (20.0, 70.0, 0.0)  # through the clump: partition 0, run 3
-    s[8] = (150.0, -30.0, 0.0)     # outside the box
-    e = s.copy()
-    e[:, 2] = 100.0
-    e[9] = s[9]                    # zero length
-    old = tuning.get_tuning()
-    try:
-        tuning.set_tuning(ray_parts=8)
-        order = estimated_order(pc, torch.from_numpy(s), torch.from_numpy(e)).numpy()
-    finally:
-        tuning.set_tuning(**old)
-    assert np.array_equal(np.sort(order), np.arange(n))
-    assert list(order[:64]) == list(range(192, 256))

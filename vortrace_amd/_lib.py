@@ -71,6 +71,7 @@ SIGNATURES = {
     "vt_cloud_set_fields": (I, [P, P, P]),
     "vt_cloud_set_ray_rows": (I, [P, C.c_int64]),
     "vt_cloud_set_ray_order": (I, [P, P, C.c_int64, P]),
+    "vt_cloud_order_rays": (I, [P, P, P, P, C.c_int64, C.c_int64, C.c_int64, P, P, P]),
     "vt_cloud_save_tree": (I, [P, P, S, C.POINTER(S)]),
     "vt_cloud_load_tree": (I, [P, P, S, I]),
     "vt_slice": (I, [P, P, S, S, C.c_double, P, P]),

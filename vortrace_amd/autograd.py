@@ -82,12 +82,14 @@ def _rays_key(pos_start, pos_end):
 
 
 class _RayState:
-    """What a forward pass leaves for the next pass over the same rays."""
-    __slots__ = ("estimate", "order")
+    """What a forward pass leaves for the next pass over the same rays: the
+    segment total (record pool size) and, when ordering pays, the per-ray
+    segment counts the next pass orders its rays by."""
+    __slots__ = ("estimate", "counts")
 
-    def __init__(self, estimate, order):
+    def __init__(self, estimate, counts):
         self.estimate = estimate
-        self.order = order
+        self.counts = counts
 
 
 _MAX_RAY_STATES = 8
@@ -195,11 +197,11 @@ class ProjectFields(torch.autograd.Function):
         records, n_records = None, -1
         # The previous pass's counts over the same rays order this pass's
         # rays longest first (speed only; any permutation gives the same bits).
-        order = state.order if state is not None else None
-        ordered = (order is not None and order.numel() == proj.ngrid
+        prev = state.counts if state is not None else None
+        ordered = (prev is not None and prev.numel() == proj.ngrid
                    and not cloud.get_ray_order_size())
         if ordered:
-            cloud.set_ray_order(order)
+            cloud.order_rays(counts=prev, row=cloud.get_ray_rows())
         try:
             if ctx.needs_input_grad[0] and state is not None:
                 lanes = cloud.lanes(mode, recorded=True)
@@ -221,10 +223,8 @@ class ProjectFields(torch.autograd.Function):
                 cloud.set_ray_order(None)
         if ctx.needs_input_grad[0]:
             lanes = cloud.lanes(mode, recorded=True)
-            states[key] = _RayState(
-                int(counts.sum().item()),
-                longest_first(counts, cloud.get_ray_rows())
-                if _order_pays(counts.numel(), lanes) else None)
+            states[key] = _RayState(int(counts.sum().item()),
+                                    counts if _order_pays(counts.numel(), lanes) else None)
             states.move_to_end(key)
             while len(states) > _MAX_RAY_STATES:
                 states.popitem(last=False)

@@ -276,6 +276,44 @@ class PointCloud:
                                                      order.numel(), _stream()))
         self._ray_order_n = order.numel()
 
+    def order_rays(self, start: torch.Tensor | None = None, end: torch.Tensor | None = None,
+                   counts: torch.Tensor | None = None, row: int = 0, block: int = 1 << 15,
+                   out: bool = False, keys: torch.Tensor | None = None):
+        """Sets the order in which the kernels take the coming calls' rays
+        (as set_ray_order) to longest first, built on the device
+        (vt_cloud_order_rays): each block of `block` fetch slots of a ray
+        partition takes its runs of 64 slots by descending key, the largest
+        of `counts` (int64 segment counts of a previous pass over the same
+        rays) over the run's rays, or, without counts, a cost estimated from
+        the rays (start, end) and the point density.  `row`: rays per image
+        row of row-major images (8x8-pixel tiles), 0 for index order.
+        Returns the order (int32 device tensor) when `out`; `keys` (int64,
+        one per run) receives the run keys.  Speed only."""
+        if not self.get_tree_built():
+            raise RuntimeError("There is currently no valid tree for this point cloud")
+        dev = self._device
+        if counts is not None:
+            if counts.dtype != torch.int64 or counts.dim() != 1 or counts.device != dev:
+                raise ValueError("counts must be a 1-D int64 tensor on the cloud's device")
+            counts = counts.contiguous()
+            n = counts.numel()
+            sp = ep = None
+        else:
+            start = device_f64(start, dev).reshape(-1, 3)
+            end = device_f64(end, dev).reshape(-1, 3)
+            if start.shape != end.shape:
+                raise ValueError("start and end must have the same shape")
+            n = start.shape[0]
+            sp, ep = _P(start.data_ptr()), _P(end.data_ptr())
+        order = torch.empty(n, dtype=torch.int32, device=dev) if out else None
+        self._ray_order_n = 0
+        _lib.check(_lib.lib().vt_cloud_order_rays(
+            self._handle(), sp, ep, _P(counts.data_ptr()) if counts is not None else None,
+            n, int(row), int(block), _P(order.data_ptr()) if out else None,
+            _P(keys.data_ptr()) if keys is not None else None, _stream()))
+        self._ray_order_n = n
+        return order
+
     def get_ray_order_size(self) -> int:
         """Ray count of the order set by set_ray_order (0: none)."""
         return getattr(self, "_ray_order_n", 0)

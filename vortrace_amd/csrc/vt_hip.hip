@@ -33,6 +33,7 @@
 #include "../../include/vortrace_hip.h"
 #include "adjoint.h"
 #include "engine.h"
+#include "order.h"
 #include "tree_build.h"
 
 #pragma clang fp contract(off)
@@ -578,6 +579,8 @@ struct vt_cloud {
   int64_t order_n = 0;
   uint32_t *d_defer = nullptr;  // rays left to the K_EXPAND-capable kernels
   size_t defer_cap = 0;
+  uint32_t *d_order_hist = nullptr;  // ray-order estimate: generators per histogram cell
+  float *d_order_grid = nullptr;     // and their cube roots (built on first use)
   int kspill = 0;
   DevCloud dc{};
   // Work buffers of the adjoint, recording and field-update calls, kept
@@ -589,8 +592,10 @@ struct vt_cloud {
                     (void *)d_err, (void *)d_stats, (void *)scratch.knn_spill,
                     (void *)scratch.walk_tag, (void *)scratch.walk_s,
                     (void *)scratch.big_id, (void *)scratch.big_d, (void *)d_defer,
-                    (void *)d_order})
+                    (void *)d_order, (void *)d_order_hist, (void *)d_order_grid})
       if (p) (void)hipFree(p);
+    d_order_hist = nullptr;
+    d_order_grid = nullptr;
     d_order = nullptr;
     order_cap = 0;
     order_n = 0;
@@ -1628,6 +1633,85 @@ int vt_cloud_set_ray_order(vt_cloud *c, const uint32_t *d_order, int64_t n, void
     HIP_OK(hipMemcpyAsync(&invalid, bad, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     if (invalid) return fail(VT_EINVAL, "ray order is not a permutation of [0, n)");
+    c->order_n = n;
+    return VT_OK;
+  });
+}
+
+int vt_cloud_order_rays(vt_cloud *c, const double *d_start, const double *d_end,
+                        const int64_t *d_counts, int64_t n, int64_t row, int64_t block,
+                        uint32_t *d_order_out, int64_t *d_run_keys, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (n < 0 || n > 0xFFFFFFFFll) return fail(VT_EINVAL, "ray count out of range [0, 2^32)");
+    if (row < 0 || row > 0x7FFFFFFF) return fail(VT_EINVAL, "row length out of range [0, 2^31)");
+    if (block < 1 || block > 64 * kOrderMaxRuns)
+      return fail(VT_EINVAL, fmt("block out of range [1, %d]", 64 * kOrderMaxRuns));
+    if (!d_counts && n > 0 && (!d_start || !d_end)) return fail(VT_EINVAL, "null argument");
+    c->order_n = 0;
+    if (n == 0) return VT_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (c->order_cap < (size_t)n) {
+      if (c->d_order) (void)hipFree(c->d_order);
+      c->d_order = nullptr;
+      c->order_cap = 0;
+      c->d_order = dev_alloc<uint32_t>((size_t)n);
+      c->order_cap = (size_t)n;
+    }
+    RunPlan P;
+    P.n = n;
+    P.parts = ray_parts();
+    P.psize = (n + P.parts - 1) / P.parts;
+    P.block = block;
+    P.per_part = (P.psize + block - 1) / block;
+    P.per_block = (block + 63) / 64;
+    P.row = (uint32_t)row;
+    const uint64_t band = 8ull * (uint64_t)row;
+    P.nfull = row ? (uint32_t)((uint64_t)n / band * band) : 0u;
+    const int64_t nblocks = (int64_t)P.parts * P.per_part;
+    const int64_t nruns = nblocks * P.per_block;
+    DevBufs B(c);
+    int64_t *keys = d_run_keys ? d_run_keys : B.get<int64_t>((size_t)nruns);
+    double lo[3], sz[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = c->subbox[2 * d];
+      sz[d] = std::max(c->subbox[2 * d + 1] - c->subbox[2 * d], 1e-300);
+    }
+    if (!d_counts && !c->d_order_grid) {
+      constexpr int cells = kOrderGrid * kOrderGrid * kOrderGrid;
+      uint32_t *hist = dev_alloc<uint32_t>(cells);
+      float *grid = nullptr;
+      try {
+        grid = dev_alloc<float>(cells);
+      } catch (...) {
+        (void)hipFree(hist);
+        throw;
+      }
+      c->d_order_hist = hist;
+      c->d_order_grid = grid;
+      HIP_OK(hipMemsetAsync(hist, 0, cells * sizeof(uint32_t), st));
+      k_order_hist<<<(unsigned)((c->n + 255) / 256), 256, 0, st>>>(
+          c->d_gen, (uint32_t)c->n, lo[0], lo[1], lo[2], sz[0], sz[1], sz[2], hist);
+      HIP_OK(hipGetLastError());
+      k_order_cbrt<<<(cells + 255) / 256, 256, 0, st>>>(hist, grid);
+      HIP_OK(hipGetLastError());
+    }
+    float flo[3], fsz[3];
+    for (int d = 0; d < 3; ++d) {
+      flo[d] = (float)lo[d];
+      fsz[d] = std::max((float)sz[d], 1e-30f);
+    }
+    const float diag = std::sqrt(fsz[0] * fsz[0] + fsz[1] * fsz[1] + fsz[2] * fsz[2]);
+    k_order_keys<<<(unsigned)((nruns + 255) / 256), 256, 0, st>>>(
+        P, nruns, d_counts, d_start, d_end, c->d_order_grid, flo[0], flo[1], flo[2], fsz[0],
+        fsz[1], fsz[2], 1.0f / diag, c->periodic, keys);
+    HIP_OK(hipGetLastError());
+    k_order_write<<<(unsigned)nblocks, 256, 0, st>>>(P, keys, c->d_order);
+    HIP_OK(hipGetLastError());
+    if (d_order_out)
+      HIP_OK(hipMemcpyAsync(d_order_out, c->d_order, (size_t)n * sizeof(uint32_t),
+                            hipMemcpyDeviceToDevice, st));
     c->order_n = n;
     return VT_OK;
   });

@@ -105,11 +105,9 @@ class ProjectionCloud:
         p = Projection(start.reshape(-1, 3), end.reshape(-1, 3))
         self._require_tree()
         with self._image_rows(shape[1]):
-            order = self._estimated_order(p, mode)
-            if order is None:
+            if not self._estimated_order(p, mode):
                 p.makeProjection(self._cloud, mode)
             else:
-                self._cloud.set_ray_order(order)
                 try:
                     p.makeProjection(self._cloud, mode)
                 finally:
@@ -221,26 +219,26 @@ class ProjectionCloud:
         mode = self._validate_reduction(reduction)
         self._require_tree()
         proj = Projection(pos_start, pos_end)
-        order = self._estimated_order(proj, mode)
-        if order is None:
+        if not self._estimated_order(proj, mode):
             return proj.project_tensor(self._cloud, mode)
-        self._cloud.set_ray_order(order)
         try:
             return proj.project_tensor(self._cloud, mode)
         finally:
             self._cloud.set_ray_order(None)
 
-    def _estimated_order(self, proj, mode=ReductionMode.Sum):
-        """Longest-first order of the rays by estimated cost (order.py) when it
-        pays (autograd._order_pays, with the lanes of the kernel the call
-        launches) and the caller set no order of its own."""
+    def _estimated_order(self, proj, mode=ReductionMode.Sum) -> bool:
+        """Sets a longest-first order of the rays by estimated cost
+        (PointCloud.order_rays, built on the device) when it pays
+        (autograd._order_pays, with the lanes of the kernel the call
+        launches) and the caller set no order of its own; returns whether it
+        did (the caller then clears it after its call)."""
         from .autograd import _order_pays
-        from .order import estimated_order
         if self._cloud.get_ray_order_size() or not _order_pays(proj.ngrid,
                                                                self._cloud.lanes(mode)):
-            return None
+            return False
         s, e = proj._rays(self._cloud)
-        return estimated_order(self, s, e, self._cloud.get_ray_rows())
+        self._cloud.order_rays(s, e, row=self._cloud.get_ray_rows())
+        return True
 
     def batch_projection_tensor(self, views, *, reduction: str = "integrate"):
         """Projects several ray sets -- views of any sizes and directions,
