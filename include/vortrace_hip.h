@@ -109,6 +109,21 @@ int vt_query_knn(const vt_cloud *cloud, const double *d_q, size_t nq, int k, int
  * VT_VOLUME and nfields otherwise (src/reduction.cpp:98-101). */
 int vt_project(const vt_cloud *cloud, const double *d_start, const double *d_end, size_t nrays,
                int mode, double *d_out, void *stream);
+/* vt_project that allocates nothing: every per-call device buffer (the
+ * deferred-ray list, the kNN stacks below their LDS windows, the walk's
+ * pending split points, the K_EXPAND buffers) comes out of one static
+ * 128 MiB device array that the library holds on each GPU it runs on for the
+ * life of the process, and the call's error record and statistics are
+ * cleared by a kernel, not a memset.  The caller passes the inputs and the
+ * output as for vt_project; the persistent grid is as wide as the array
+ * holds buffers for.  At most 4,194,304 rays per call (VT_EINVAL beyond).
+ * Calls that overlap in time on one GPU share the array and race.  No
+ * reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). */
+int vt_project_noalloc(const vt_cloud *cloud, const double *d_start, const double *d_end,
+                       size_t nrays, int mode, double *d_out, void *stream);
+
 /* vt_project that also writes every ray's segment count to d_counts (nrays,
  * the counts of vt_project_count) in the same pass; d_counts may be NULL.
  * No reference counterpart: the forward pass of a differentiable projection

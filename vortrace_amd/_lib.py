@@ -66,6 +66,7 @@ SIGNATURES = {
     "vt_cloud_tree_info": (I, [P, C.POINTER(TreeInfo)]),
     "vt_query_knn": (I, [P, P, S, I, P, P, P]),
     "vt_project": (I, [P, P, P, S, I, P, P]),
+    "vt_project_noalloc": (I, [P, P, P, S, I, P, P]),
     "vt_project_count": (I, [P, P, P, S, P, P]),
     "vt_project_segments": (I, [P, P, P, S, I, P, P, P, P, P, P]),
     "vt_cloud_set_fields": (I, [P, P, P]),

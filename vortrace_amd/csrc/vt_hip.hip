@@ -1076,6 +1076,156 @@ int64_t scan_counts(const int64_t *cnt, size_t nrays, int64_t *offsets, DevBufs 
 
 }  // namespace
 
+// ------------------------------------------------ no-allocation projection ----
+// vt_project_noalloc runs vt_project's kernels with every per-call buffer --
+// the deferred-ray list, the kNN stacks below their LDS windows, the walk's
+// pending split points and the K_EXPAND buffers -- carved out of one static
+// device array, so that the call allocates nothing.  The array is the
+// library's own per device (the loader places it on every GPU the library
+// runs on, for the life of the process); calls on one GPU share it, so calls
+// that overlap in time on one GPU race.  Nothing in it needs clearing: every
+// buffer is written before it is read.
+
+namespace vt {
+
+constexpr size_t kNoAllocBytes = size_t(128) << 20;  // 128 MiB per device
+constexpr int64_t kNoAllocMaxRays = int64_t(1) << 22;  // deferred list: 16 MiB at most
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Zeroes a call's error record and statistics (a kernel, so the call issues
+// no memset).
+__global__ void k_clear_call(ErrRec *err, Stats *stats) {
+  uint32_t *e = reinterpret_cast<uint32_t *>(err);
+  uint32_t *s = reinterpret_cast<uint32_t *>(stats);
+  constexpr int ne = sizeof(ErrRec) / 4, ns = sizeof(Stats) / 4;
+  static_assert(sizeof(ErrRec) % 4 == 0 && sizeof(Stats) % 4 == 0, "word-sized records");
+  for (int i = threadIdx.x; i < ne + ns; i += blockDim.x) {
+    if (i < ne)
+      e[i] = 0u;
+    else
+      s[i - ne] = 0u;
+  }
+}
+
+}  // namespace vt
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+unsigned char *g_noalloc_base[kMaxDevices] = {};
+
+// The device address of `scratch` on device `dev` (current), looked up once
+// per device.  Host code uses only this address, never the array's name.
+unsigned char *noalloc_base(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw HipError("device index out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_base[dev]) {
+    void *p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(vt::scratch)));
+    if (!p) throw HipError("no device address for the no-allocation scratch array");
+    g_noalloc_base[dev] = static_cast<unsigned char *>(p);
+  }
+  return g_noalloc_base[dev];
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Splits the array for a call of nrays rays on cloud c into the deferred
+// list and the per-lane buffers of `grid` workgroups: as many workgroups as
+// fit, at most full_grid.  Returns 0 when not even one fits.
+int noalloc_layout(const vt_cloud *c, unsigned char *base, int64_t nrays, int full_grid,
+                   Scratch &s, uint32_t *&defer) {
+  const size_t kspill = (size_t)c->kspill;
+  const size_t defer_b = align256((size_t)nrays * sizeof(uint32_t));
+  const size_t per_wg = (size_t)kBlock * (kspill * sizeof(uint4) +
+                                         (size_t)kWalkCap * (sizeof(uint32_t) + sizeof(double))) +
+                        (size_t)(kBlock / 64) * 4 * kMaxQueryK * (sizeof(uint32_t) + sizeof(double));
+  const size_t slack = 5 * 256;  // alignment of the five per-lane arrays
+  if (defer_b + slack >= kNoAllocBytes) return 0;
+  const int grid = (int)std::min<size_t>((size_t)full_grid, (kNoAllocBytes - defer_b - slack) / per_wg);
+  if (grid < 1) return 0;
+  const size_t lanes = (size_t)grid * kBlock, waves = lanes / 64;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char *p = base + off;
+    off += align256(bytes);
+    return p;
+  };
+  defer = reinterpret_cast<uint32_t *>(take((size_t)nrays * sizeof(uint32_t)));
+  s = Scratch{};
+  s.kspill = (int)kspill;
+  s.wcap = kWalkCap;
+  s.lanes = (int)lanes;
+  s.knn_spill = reinterpret_cast<uint4 *>(take(kspill * lanes * sizeof(uint4)));
+  s.walk_tag = reinterpret_cast<uint32_t *>(take((size_t)kWalkCap * lanes * sizeof(uint32_t)));
+  s.walk_s = reinterpret_cast<double *>(take((size_t)kWalkCap * lanes * sizeof(double)));
+  s.big_id = reinterpret_cast<uint32_t *>(take(waves * 4 * kMaxQueryK * sizeof(uint32_t)));
+  s.big_d = reinterpret_cast<double *>(take(waves * 4 * kMaxQueryK * sizeof(double)));
+  if (off > kNoAllocBytes) return 0;  // cannot happen: per_wg and slack bound it
+  s.walk_batch = walk_batch();
+  s.leaf_batch = leaf_batch();
+  s.parts = ray_parts();
+  return grid;
+}
+
+// vt_project over the no-allocation array: the fast kernel, then the
+// K_EXPAND-capable one over the rays it deferred, each on as many
+// workgroups as the array holds buffers for.
+template <int MODE, int NF, bool PER>
+int project_noalloc(vt_cloud *c, const double *d_s, const double *d_e, int64_t n, double *out,
+                    int nout, hipStream_t st) {
+  unsigned char *base = noalloc_base(c->device);
+  auto kfast = k_project<MODE, NF, PER, false>;
+  auto kslow = k_project_x<MODE, PER, false>;
+  const int full = std::max(grid_for(kfast, INT64_MAX / 2, c->device),
+                            grid_for(kslow, INT64_MAX / 2, c->device));
+  // One layout for both launches: the per-lane arrays are strided by the
+  // lanes of `fit` workgroups; each launch runs at most that many.
+  Scratch s;
+  uint32_t *defer = nullptr;
+  const int fit = noalloc_layout(c, base, n, full, s, defer);
+  if (fit < 1)
+    return fail(VT_EINVAL, fmt("vt_project_noalloc: the tree's depth leaves no room for one "
+                               "workgroup's buffers in the %zu MiB scratch array",
+                               kNoAllocBytes >> 20));
+  auto prepare = [&](auto kernel, int64_t nr, bool own_order) {
+    const int grid = std::min(fit, grid_for(kernel, INT64_MAX / 2, c->device));
+    const int64_t lanes = (int64_t)grid * kBlock;
+    s.fetch_lanes =
+        (int)std::min<int64_t>(64, std::max<int64_t>(1, (64 * nr + lanes - 1) / lanes));
+    s.tile_row = c->ray_row;
+    s.ray_order = (own_order && c->order_n == nr && nr > 0) ? c->d_order : nullptr;
+    k_clear_call<<<1, 64, 0, st>>>(c->d_err, c->d_stats);
+    HIP_OK(hipGetLastError());
+    return grid;
+  };
+  {
+    const int grid = prepare(kfast, n, true);
+    kfast<<<grid, kBlock, 0, st>>>(c->dc, s, d_s, d_e, n, defer, out, nout, nullptr, RecPool{},
+                                   c->d_err, c->d_stats);
+    HIP_OK(hipGetLastError());
+    const int rc = finish_call(c, st, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  const int64_t nd = g_stats.deferred_rays;
+  if (nd == 0) return VT_OK;
+  const int grid = prepare(kslow, nd, false);
+  kslow<<<grid, kBlock, 0, st>>>(c->dc, s, d_s, d_e, nd, (const uint32_t *)defer, out, nout,
+                                 nullptr, RecPool{}, c->d_err, c->d_stats);
+  HIP_OK(hipGetLastError());
+  return finish_call(c, st, nullptr, nullptr, true);
+}
+
+template <int MODE, int NF>
+int project_noalloc_any(vt_cloud *c, const double *s, const double *e, int64_t n, double *out,
+                        int nout, hipStream_t st) {
+  return c->periodic ? project_noalloc<MODE, NF, true>(c, s, e, n, out, nout, st)
+                     : project_noalloc<MODE, NF, false>(c, s, e, n, out, nout, st);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *vt_last_error(void) { return g_err.c_str(); }
@@ -1416,6 +1566,38 @@ int vt_project_counted(const vt_cloud *cc, const double *d_start, const double *
     if (mode == VT_MAX) return project<VT_MAX, 0>(c, d_start, d_end, n, d_out, nout, d_counts, st);
     if (mode == VT_MIN) return project<VT_MIN, 0>(c, d_start, d_end, n, d_out, nout, d_counts, st);
     return project<VT_VOLUME, 4>(c, d_start, d_end, n, d_out, nout, d_counts, st);
+  });
+}
+
+int vt_project_noalloc(const vt_cloud *cc, const double *d_start, const double *d_end,
+                       size_t nrays, int mode, double *d_out, void *stream) {
+  return guarded([&]() -> int {
+    int rc = check_ready(cc);
+    if (!rc) rc = check_mode(cc, mode);
+    if (rc) return rc;
+    if (nrays > (size_t)kNoAllocMaxRays)
+      return fail(VT_EINVAL, fmt("vt_project_noalloc: %zu rays; at most %lld per call (its "
+                                 "buffers live in a fixed %zu MiB device array)",
+                                 nrays, (long long)kNoAllocMaxRays, kNoAllocBytes >> 20));
+    vt_cloud *c = const_cast<vt_cloud *>(cc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (nrays == 0) return VT_OK;
+    const int nout = mode == VT_VOLUME ? 3 : (int)c->nf;
+    const int64_t n = (int64_t)nrays;
+    if (mode == VT_SUM) {
+      switch (c->nf) {
+        case 1: return project_noalloc_any<VT_SUM, 1>(c, d_start, d_end, n, d_out, nout, st);
+        case 2: return project_noalloc_any<VT_SUM, 2>(c, d_start, d_end, n, d_out, nout, st);
+        case 3: return project_noalloc_any<VT_SUM, 3>(c, d_start, d_end, n, d_out, nout, st);
+        case 4: return project_noalloc_any<VT_SUM, 4>(c, d_start, d_end, n, d_out, nout, st);
+        default: return project_noalloc_any<VT_SUM, 0>(c, d_start, d_end, n, d_out, nout, st);
+      }
+    }
+    if (mode == VT_MAX)
+      return project_noalloc_any<VT_MAX, 0>(c, d_start, d_end, n, d_out, nout, st);
+    if (mode == VT_MIN)
+      return project_noalloc_any<VT_MIN, 0>(c, d_start, d_end, n, d_out, nout, st);
+    return project_noalloc_any<VT_VOLUME, 4>(c, d_start, d_end, n, d_out, nout, st);
   });
 }
 
