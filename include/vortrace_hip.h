@@ -269,12 +269,15 @@ int vt_grid_rays(const vt_view *views, size_t nviews, double *d_start, double *d
 
 /* Launch parameters of the ray kernels (process-wide): lanes of a wave that
  * run their walk steps together, lanes that run their leaf steps together
- * (both 1..64), and ray partitions (1, or 8: one per XCD).  Results do not
- * depend on them, only speed.  No reference counterpart (the reference's
+ * (both 1..64), ray partitions (1, or 8: one per XCD), and help_events
+ * (0..128): lanes with no ray left walk intervals of other lanes' rays of
+ * their wave (sum, max, min and count kernels), recording up to this many
+ * boundaries of one interval before they hand it back (0: off).  Results do
+ * not depend on them, only speed.  No reference counterpart (the reference's
  * OpenMP loop has schedule(dynamic,256) fixed, src/projection.cpp:43);
  * vortrace_amd.tuning.autotune picks them. */
 typedef struct {
-  int walk_batch, leaf_batch, ray_parts;
+  int walk_batch, leaf_batch, ray_parts, help_events;
 } vt_tuning;
 int vt_set_tuning(const vt_tuning *t);
 int vt_get_tuning(vt_tuning *t);
@@ -289,11 +292,14 @@ int vt_cloud_lanes(const vt_cloud *cloud, int mode, int recorded, int64_t *lanes
 
 /* Per-call statistics of the last vt_project* call on this thread: total
  * segments (cell crossings), kNN queries run, the deepest walk and kNN
- * stacks, K_EXPAND searches, rays re-run by the K_EXPAND-capable kernel, and
- * the kernels' time. */
+ * stacks, K_EXPAND searches, rays re-run by the K_EXPAND-capable kernel, the
+ * kernels' time, and the intervals of rays walked by other lanes than the
+ * ray's own (vt_tuning.help_events) and of those the ones handed back
+ * unwalked. */
 typedef struct {
   int64_t segments, queries, max_walk_depth, max_knn_depth, slow_queries, deferred_rays;
   float kernel_ms;
+  int64_t intervals, intervals_back;
 } vt_call_stats;
 int vt_last_call_stats(vt_call_stats *out);
 

@@ -54,19 +54,21 @@ def test_tuning_changes_speed_only(clustered):
     saved = vt.tuning.get_tuning()
     try:
         ref = pc.projection(s, e, reduction="sum")
-        for wb, lb, rp in [(1, 1, 1), (64, 64, 8), (8, 40, 1)]:
-            vt.tuning.set_tuning(wb, lb, rp)
+        for wb, lb, rp, he in [(1, 1, 1, 0), (64, 64, 8, 2), (8, 40, 1, 128)]:
+            vt.tuning.set_tuning(wb, lb, rp, he)
             np.testing.assert_array_equal(pc.projection(s, e, reduction="sum"), ref,
-                                          err_msg=f"tuning {wb} {lb} {rp}")
+                                          err_msg=f"tuning {wb} {lb} {rp} {he}")
         out = vt.tuning.autotune(pc, s, e, reduction="sum", stride=2,
                                  walk_batches=(16, 32), leaf_batches=(16, 32),
                                  ray_parts=(1, 8), repeats=1)
-        assert out["best"] == {k: out["best"][k] for k in ("walk_batch", "leaf_batch",
-                                                          "ray_parts")}
+        assert set(out["best"]) == {"walk_batch", "leaf_batch", "ray_parts", "help_events"}
+        assert out["best"]["help_events"] == 128
         assert len(out["results"]) == 8
         assert out["kernel_ms"] == min(r["kernel_ms"] for r in out["results"])
         np.testing.assert_array_equal(pc.projection(s, e, reduction="sum"), ref)
         with pytest.raises(ValueError, match="tuning out of range"):
             vt.tuning.set_tuning(walk_batch=0)
+        with pytest.raises(ValueError, match="help_events in"):
+            vt.tuning.set_tuning(help_events=129)
     finally:
         vt.tuning.set_tuning(**saved)

@@ -292,3 +292,27 @@ def test_longest_first_order(n, row, parts, block):
             expect = [fetch[sl] for k in ranked for sl in runs[k]]
             assert list(o[lo:hi]) == expect
 
+
+
+def test_default_tuning_and_environment_overrides():
+    """Launch parameters at load: the defaults with no VT_* variable set, and
+    each variable overriding its own parameter (out-of-range values ignored)."""
+    import json
+    import subprocess
+    import sys
+    code = ("import json, vortrace_amd as vt; "
+            "print(json.dumps(vt.tuning.get_tuning()))")
+
+    def load(**env_vars):
+        env = {k: v for k, v in os.environ.items() if not k.startswith("VT_")}
+        env.update(env_vars)
+        out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, check=True,
+                             capture_output=True, text=True, timeout=120).stdout
+        return json.loads(out.strip().splitlines()[-1])
+
+    assert load() == {"walk_batch": 32, "leaf_batch": 32, "ray_parts": 8, "help_events": 128}
+    assert load(VT_HELP_EVENTS="0")["help_events"] == 0
+    assert load(VT_HELP_EVENTS="7", VT_WALK_BATCH="16") == {
+        "walk_batch": 16, "leaf_batch": 32, "ray_parts": 8, "help_events": 7}
+    assert load(VT_HELP_EVENTS="129", VT_RAY_PARTS="1")["help_events"] == 128
+    assert load(VT_RAY_PARTS="1")["ray_parts"] == 1

@@ -22,7 +22,8 @@ class TreeInfo(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("walk_batch", C.c_int), ("leaf_batch", C.c_int), ("ray_parts", C.c_int)]
+    _fields_ = [("walk_batch", C.c_int), ("leaf_batch", C.c_int), ("ray_parts", C.c_int),
+                ("help_events", C.c_int)]
 
 
 class View(C.Structure):
@@ -36,7 +37,8 @@ class CallStats(C.Structure):
     _fields_ = [("segments", C.c_int64), ("queries", C.c_int64),
                 ("max_walk_depth", C.c_int64), ("max_knn_depth", C.c_int64),
                 ("slow_queries", C.c_int64), ("deferred_rays", C.c_int64),
-                ("kernel_ms", C.c_float)]
+                ("kernel_ms", C.c_float), ("intervals", C.c_int64),
+                ("intervals_back", C.c_int64)]
 
 
 WARN_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)

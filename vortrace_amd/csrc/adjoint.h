@@ -38,6 +38,7 @@ struct SegSink {
 
 struct SegIO {
   using Sink = SegSink;
+  static constexpr bool kHelp = false;
   const int64_t *offsets;
   uint32_t *slot;
   double *ds;
@@ -78,6 +79,7 @@ constexpr int kRecChunk = 32;
 // values: the backward pass writes both to the same place.
 template <class InnerIO>
 struct RecordIO {
+  static constexpr bool kHelp = false;
   struct Sink {
     typename InnerIO::Sink in;
     RecPool pool;

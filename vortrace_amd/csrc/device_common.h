@@ -44,6 +44,7 @@ enum : int {
                            // periodic images merged
   kErrNonFinite = 10,      // a query found no generator (non-finite point)
   kErrFillOverflow = 11,   // fill pass produced more segments than counted
+  kErrHelpWait = 12,       // a lane waited too long for another lane of its wave
   // Not an error: the ray needs the K_EXPAND classification, which the fast
   // kernels leave to a second launch over the deferred rays.
   kDeferRay = 64,
@@ -67,6 +68,8 @@ struct Stats {
   unsigned long long ray_counter;  // work queue head (query kernels)
   unsigned long long deferred;     // rays appended to the deferred list
   unsigned long long part_counter[8];  // ray engine: work queue head per partition
+  unsigned long long intervals;        // ray engine: intervals given to other lanes
+  unsigned long long intervals_back;   // ... of which came back unwalked
 };
 
 // The ray engine splits its rays into kParts contiguous partitions, one per
@@ -107,6 +110,10 @@ struct Scratch {
   int fetch_lanes;     // ray engine: lanes of each wave that take rays (1..64)
   uint32_t tile_row;   // ray engine: rays per image row for the 8x8-tile ray order (0 = off)
   const uint32_t *ray_order;  // ray engine: the order fetch slots take rays in (null = own order)
+  // ray engine, intervals walked for another lane of the wave (null: off):
+  uint4 *help_ev;      // [help_cap][T] events of the interval this lane walks
+  uint4 *help_steal;   // [2 * help_steal_cap][T] intervals this lane gave away
+  int help_cap, help_steal_cap;
 };
 
 constexpr int kMaxQueryK = 128;

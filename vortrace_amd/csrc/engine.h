@@ -414,7 +414,9 @@ enum : int {
   kPhFetch = 0,  // needs a ray
   kPhWalk = 1,   // a search has finished: walk step due
   kPhSearch = 2,
-  kPhDone = 3,   // no rays left
+  kPhDone = 3,   // nothing to do (no rays left): free to walk another lane's interval
+  kPhReplay = 4,  // own list exhausted, waiting to replay intervals given away
+  kPhHelped = 5,  // a given interval is walked, waiting until its giver replayed it
 };
 
 // The 8x8-tile ray order: with the rays the pixels of row-major images whose
@@ -441,13 +443,78 @@ struct EngineCounters {
   int maxw = 0;
 };
 
+// ---- walking another lane's intervals (lanes of one wave) -------------------
+// A lane with nothing left to fetch takes the bottom pending pair of a busy
+// lane of its wave (Walker::steal_bottom) and walks that interval; it records
+// its case-2 boundaries and zero-width switches as events, which the giver
+// replays, most recent gift first, once its own list is exhausted, before it
+// closes the ray (or, for a lane that is itself walking a gift, before it
+// reports its own events).  Replayed in order, the events give the giver's
+// sink exactly the segments of the sequential walk.  Lanes exchange only
+// registers (wave shuffles) and their own device buffers; events are read
+// through the device-coherent path.
+constexpr uint32_t kEvBoundary = 1u, kEvSwitch = 2u;
+// Status a lane walking a gift publishes: state (3 bits), events (12 bits),
+// walk iterations (17 bits, saturated).
+enum : uint32_t { kHsIdle = 0, kHsWork = 1, kHsDone = 2, kHsOver = 3, kHsDefer = 4 };
+constexpr uint32_t kHsMaxEvents = 4095u, kHsMaxIter = 131071u;
+__device__ __forceinline__ uint32_t hs_pack(uint32_t st, uint32_t nev, int it) {
+  const uint32_t i = it < 0 ? 0u : ((uint32_t)it > kHsMaxIter ? kHsMaxIter : (uint32_t)it);
+  return st | (min(nev, kHsMaxEvents) << 3) | (i << 15);
+}
+// Rounds a lane waits for another before it gives up (a guard against a
+// protocol error, which would otherwise spin forever).
+constexpr int kMaxWaitRounds = 1 << 24;
+
+// The sink the walk writes to: the IO's own sink for a lane walking its own
+// ray, the event buffer for a lane walking a gift.
+template <class Inner>
+struct LaneSink {
+  Inner in;
+  uint4 *ev;  // this lane's event buffer, entries strided by `stride`
+  int stride, cap;
+  uint32_t nev;
+  bool helper, over;
+  __device__ __forceinline__ void seg(uint32_t cell, double a, double b) { in.seg(cell, a, b); }
+  __device__ __forceinline__ void put(uint32_t type, double s, uint32_t cell) {
+    if ((int)nev < cap) {
+      const uint64_t b = (uint64_t)__double_as_longlong(s);
+      gst(ev + (size_t)nev * stride, make_uint4(cell, type, (uint32_t)b, (uint32_t)(b >> 32)));
+      ++nev;
+    } else {
+      over = true;
+    }
+  }
+  // A case-2 boundary at s closing `open` (entered at `entry`) and opening `next`.
+  __device__ __forceinline__ void close(uint32_t open, double entry, double s, uint32_t next) {
+    if (helper)
+      put(kEvBoundary, s, next);
+    else
+      in.seg(open, entry, s);
+  }
+  // A zero-width step: the open cell becomes `next` without a boundary.
+  __device__ __forceinline__ void sw(uint32_t next) {
+    if (helper) put(kEvSwitch, 0.0, next);
+  }
+};
+
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
+  for (int i = 0; i < n; ++i) m &= m - 1;
+  return __ffsll((unsigned long long)m) - 1;
+}
+
+// Lane roles.
+enum : int { kRoleIdle = 0, kRoleOwner = 1, kRoleHelper = 2 };
+
 // Runs rays through the walk, handing segments to the sinks IO provides:
 // IO::Sink, io.begin(ray, sink), io.end(ray, sink) -> error code (kErrNone or
 // a failure of the ray's output).  The rays are [0, nrays), or ray_list[0,
 // nrays) when ray_list is given.  Without kExpand, a ray that needs the
 // K_EXPAND classification is abandoned and appended to `defer` (its outputs
 // are rewritten by the launch that runs the deferred rays with kExpand).
-template <bool kPer, bool kExpand, class IO>
+// With kHelp (and help buffers in S), idle lanes walk intervals of busy lanes
+// of their wave; without it the helping code is compiled out.
+template <bool kPer, bool kExpand, bool kHelp, class IO>
 __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
                                            const double *start, const double *end,
                                            int64_t nrays, const uint32_t *ray_list,
@@ -455,9 +522,26 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
                                            KStack &st, EngineCounters &cnt) {
   const int gtid = blockIdx.x * kBlock + threadIdx.x;
   const size_t wbase = (size_t)gtid;
+  const int lane = (int)(threadIdx.x & 63);
+  const size_t wave0 = wbase - (size_t)lane;  // first lane of this wave
   Walker<kPer> W;
   Search<kPer> Q;
-  typename IO::Sink sk;
+  LaneSink<typename IO::Sink> sk;
+  sk.ev = S.help_ev ? S.help_ev + wbase : nullptr;
+  sk.stride = S.lanes;
+  sk.cap = S.help_cap;
+  sk.nev = 0;
+  sk.helper = false;
+  sk.over = false;
+  const bool can_help = kHelp && S.help_ev != nullptr;
+  int role = kRoleIdle;
+  int boss = 0, my_slot = 0;  // a gift's giver (lane) and its place in the giver's list
+  uint32_t boss_epoch = 0;
+  uint32_t epoch = 0;     // changes whenever this lane's work ends or is abandoned
+  int nsteal = 0;         // gifts of the current work not replayed yet
+  int extra_it = 0;       // walk iterations of replayed gifts
+  uint32_t hstat = kHsIdle;
+  int waited = 0;
   int ph = kPhFetch;
   int64_t ray = 0;
   const int batch = S.walk_batch;
@@ -477,35 +561,149 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
       WalkErr e{{0.0, 0.0, 0.0}, {0u, 0u}};
       int rc = kErrNone;
       bool query = false;
+      bool ended = false;  // the lane's list is exhausted (its own part of the work)
       double q[3];
       int k = 1;
       double bound = DBL_MAX;
+      bool published = false;  // this lane reported the end of a gift's walk
       if (ph == kPhWalk) {
         uint32_t ids[3];
         double r2[3];
         if (Q.finish(C, st, ids, r2)) {
           ++cnt.nq;
-          rc = W.template consume<kExpand>(C, S, wbase, (int)threadIdx.x, gtid, ids, r2, sk, cnt.maxw, cnt.nslow, e);
-          if (!rc && W.state < kStFinal) rc = W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e);
-          if (!rc && W.state == kStFinal) {
-            sk.seg(W.open_cell, W.entry_s, W.sc);
-            W.state = kStDone;
+          rc = W.template consume<kExpand>(C, S, wbase, (int)threadIdx.x, gtid, ids, r2, sk,
+                                           cnt.maxw, cnt.nslow, e);
+          if (!rc && W.state < kStFinal) rc = W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e, sk);
+          if (!rc && W.state == kStFinal) ended = true;
+          // A gift whose events no longer fit goes back to its giver whole:
+          // stop walking it (and drop this lane's own gifts of it).
+          if (!rc && role == kRoleHelper && sk.over) {
+            ended = true;
+            nsteal = 0;
           }
-          if (!rc && W.state == kStDone && st.overflow) rc = kErrKnnStack;
-          if (!rc && W.state == kStDone) rc = io.end(ray, sk);
-          if (!kExpand && rc == kDeferRay) {
-            const unsigned long long slot = atomicAdd(&stats->deferred, 1ULL);
-            gst(defer + slot, (uint32_t)ray);
-            ph = kPhFetch;
-          } else if (rc || W.state == kStDone) {
-            if (rc) report_error(err, rc, ray, e.v[0], e.v[1], e.v[2], e.id[0], e.id[1]);
-            cnt.nseg += sk.n;
-            ph = kPhFetch;
-          } else {
-            query = true;
-          }
+          if (!rc && !ended && W.state != kStDone) query = true;
         } else {
           ph = kPhSearch;  // the next image search has started
+        }
+      }
+      // A lane waiting for its most recent gift: replay it once it is walked.
+      if constexpr (kHelp) {
+        uint4 rec = make_uint4(0u, 0u, 0u, 0u);
+        if (ph == kPhReplay) rec = gld(S.help_steal + (size_t)(2 * nsteal - 2) * S.lanes + wbase);
+        const uint32_t hs = __shfl(hstat, (int)rec.x);
+        if (ph == kPhReplay) {
+          const uint32_t hst = hs & 7u;
+          if (hst == kHsDone) {
+            const uint32_t nev = (hs >> 3) & kHsMaxEvents;
+            const uint4 *ev = S.help_ev + wave0 + rec.x;
+            for (uint32_t i = 0; i < nev; ++i) {
+              const uint64_t *p = reinterpret_cast<const uint64_t *>(ev + (size_t)i * S.lanes);
+              const uint64_t a = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              const uint64_t b =
+                  __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              const uint32_t cell = (uint32_t)a;
+              const double s = __longlong_as_double((long long)b);
+              if ((uint32_t)(a >> 32) == kEvBoundary) {
+                sk.close(W.open_cell, W.entry_s, s, cell);
+                W.entry_s = s;
+              } else {
+                sk.sw(cell);
+              }
+              W.open_cell = cell;
+            }
+            extra_it += (int)(hs >> 15);
+            --nsteal;
+            waited = 0;
+            if (W.iteration + extra_it > kMaxIter) {
+              rc = kErrMaxIter;
+            } else if (nsteal == 0) {
+              ended = true;
+            }
+          } else if (hst == kHsOver) {
+            // The gift did not fit its walker's event buffer: walk it here.
+            const uint4 sv = gld(S.help_steal + (size_t)(2 * nsteal - 1) * S.lanes + wbase);
+            --nsteal;
+            waited = 0;
+            atomicAdd(&stats->intervals_back, 1ULL);
+            W.take_back(rec.y, __longlong_as_double((long long)(((uint64_t)sv.y << 32) | sv.x)),
+                        rec.z, __longlong_as_double((long long)(((uint64_t)sv.w << 32) | sv.z)));
+            rc = W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e, sk);
+            if (!rc && W.state == kStFinal) ended = true;
+            if (!rc && !ended) query = true;
+          } else if (hst == kHsDefer) {
+            --nsteal;
+            rc = kDeferRay;
+          } else if (++waited > kMaxWaitRounds) {
+            rc = kErrHelpWait;
+          }
+        }
+      }
+      if (ended && nsteal > 0) {
+        ph = kPhReplay;
+      } else if (ended || (!rc && ph == kPhWalk && W.state == kStDone)) {
+        // This lane's work is complete.
+        if (role == kRoleHelper) {
+          hstat = hs_pack(sk.over ? kHsOver : kHsDone, sk.nev, W.iteration + extra_it);
+          published = true;
+          ph = kPhHelped;
+          waited = 0;
+        } else {
+          if (W.state == kStFinal) {
+            sk.seg(W.open_cell, W.entry_s, W.r.len);
+            W.state = kStDone;
+          }
+          rc = st.overflow ? kErrKnnStack : io.end(ray, sk.in);
+          if (rc) report_error(err, rc, ray, e.v[0], e.v[1], e.v[2], e.id[0], e.id[1]);
+          cnt.nseg += sk.in.n;
+          ph = kPhFetch;
+        }
+        ++epoch;
+      } else if (rc) {
+        // Deferred to the K_EXPAND launch, or failed: the work is abandoned
+        // (its gifts are released by the epoch change).
+        if (!kExpand && rc == kDeferRay) {
+          if (role == kRoleHelper) {
+            hstat = hs_pack(kHsDefer, 0u, 0);
+          } else {
+            const unsigned long long slot = atomicAdd(&stats->deferred, 1ULL);
+            gst(defer + slot, (uint32_t)ray);
+          }
+        } else {
+          report_error(err, rc, ray, e.v[0], e.v[1], e.v[2], e.id[0], e.id[1]);
+          if (role == kRoleHelper) hstat = hs_pack(kHsDone, 0u, 0);
+        }
+        if (role == kRoleHelper) {
+          published = true;
+          ph = kPhHelped;
+          waited = 0;
+        } else {
+          cnt.nseg += sk.in.n;
+          ph = kPhFetch;
+        }
+        nsteal = 0;
+        ++epoch;
+      }
+      // A lane walking a gift: released once its giver has replayed it, or
+      // when the giver abandoned its work.
+      if constexpr (kHelp) {
+        const uint32_t bep = __shfl(epoch, boss);
+        const int bns = __shfl(nsteal, boss);
+        if (role == kRoleHelper) {
+          const bool gone = bep != boss_epoch;
+          if ((ph == kPhHelped && (gone || bns <= my_slot)) ||
+              (gone && (ph == kPhSearch || ph == kPhWalk || ph == kPhReplay)) ||
+              (ph == kPhHelped && ++waited > kMaxWaitRounds)) {
+            if (ph == kPhHelped && waited > kMaxWaitRounds)
+              report_error(err, kErrHelpWait, ray, 0.0, 0.0, 0.0, 0u, 0u);
+            if (ph != kPhHelped) {
+              ++epoch;  // abandoned mid-walk: release this lane's own gifts
+              nsteal = 0;
+              query = false;
+            }
+            role = kRoleIdle;
+            hstat = kHsIdle;
+            ph = kPhDone;
+          }
         }
       }
       if (ph == kPhFetch) {
@@ -525,15 +723,87 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
         if (ray < 0 ||
             __hip_atomic_load(&err->flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
           ph = kPhDone;
+          role = kRoleIdle;
         } else {
           if (ray_list) ray = (int64_t)gld(ray_list + ray);
           else if (tile_n) ray = (int64_t)tile_ray((uint32_t)ray, tile_w, tile_n);
+          role = kRoleOwner;
+          nsteal = 0;
+          extra_it = 0;
+          sk.helper = false;
+          sk.over = false;
+          sk.nev = 0;
           W.begin(start + 3 * ray, end + 3 * ray);
-          io.begin(ray, sk);
-          W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e);  // kStStart: the ray's start
+          io.begin(ray, sk.in);
+          W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e, sk);  // kStStart: the ray's start
           query = true;
         }
       }
+      // Lanes with nothing to do take the bottom pending pair of busy lanes of
+      // their wave: the k-th idle lane the k-th lane that can give one.
+      const uint64_t idle = can_help ? __ballot(ph == kPhDone && !query) : 0ull;
+      if (kHelp && idle) {
+        // A lane that is searching, or has just prepared its next search,
+        // can give: a search reads only the top pair of the pending list.
+        const bool can = (ph == kPhSearch || query) && role != kRoleIdle && W.can_give() &&
+                         nsteal < S.help_steal_cap && W.wsp + 4 < S.wcap;
+        const uint64_t donors = __ballot(can);
+        if (donors) {
+          const uint64_t below = (1ull << lane) - 1ull;
+          const int nidle = __popcll(idle), ndon = __popcll(donors);
+          const bool take = ((idle >> lane) & 1ull) && __popcll(idle & below) < ndon;
+          const bool give = ((donors >> lane) & 1ull) && __popcll(donors & below) < nidle;
+          int src = lane;
+          uint32_t pc = 0, qc = 0;
+          double ps = 0.0, qs = 0.0;
+          int slot = 0;
+          if (take) src = nth_set_bit(donors, __popcll(idle & below));
+          if (give) {
+            const int to = nth_set_bit(idle, __popcll(donors & below));
+            W.steal_bottom(S, wbase, pc, ps, qc, qs);
+            const uint64_t pb = (uint64_t)__double_as_longlong(ps);
+            const uint64_t qb = (uint64_t)__double_as_longlong(qs);
+            uint4 *rec = S.help_steal + (size_t)(2 * nsteal) * S.lanes + wbase;
+            gst(rec, make_uint4((uint32_t)to, pc, qc, 0u));
+            gst(rec + S.lanes, make_uint4((uint32_t)pb, (uint32_t)(pb >> 32), (uint32_t)qb,
+                                          (uint32_t)(qb >> 32)));
+            slot = nsteal++;
+            atomicAdd(&stats->intervals, 1ULL);
+          }
+          const uint32_t t_pc = __shfl(pc, src), t_qc = __shfl(qc, src);
+          const double t_ps = __shfl(ps, src), t_qs = __shfl(qs, src);
+          const uint32_t t_ray = __shfl((uint32_t)ray, src), t_ep = __shfl(epoch, src);
+          const int t_slot = __shfl(slot, src);
+          if (take) {
+            role = kRoleHelper;
+            boss = src;
+            boss_epoch = t_ep;
+            my_slot = t_slot;
+            ray = (int64_t)t_ray;
+            ++epoch;
+            nsteal = 0;
+            extra_it = 0;
+            waited = 0;
+            sk.helper = true;
+            sk.over = false;
+            sk.nev = 0;
+            W.begin_interval(start + 3 * ray, end + 3 * ray, t_pc, t_ps, t_qc, t_qs);
+            hstat = kHsWork;
+            const int rc2 = W.prepare(C, S, wbase, end + 3 * ray, q, k, bound, e, sk);
+            if (rc2) report_error(err, rc2, ray, e.v[0], e.v[1], e.v[2], e.id[0], e.id[1]);
+            if (rc2 || W.state == kStFinal) {  // nothing to walk (zero-width) or failed
+              hstat = hs_pack(sk.over ? kHsOver : kHsDone, sk.nev, W.iteration);
+              published = true;
+              ph = kPhHelped;
+            } else {
+              query = true;
+            }
+          }
+        }
+      }
+      // A gift's events are in device memory before its giver can see its
+      // status (read through a shuffle in a later iteration).
+      if (kHelp && __ballot(published)) __threadfence();
       if (query) {
         Q.start(C, st, q, k, bound);
         ph = kPhSearch;

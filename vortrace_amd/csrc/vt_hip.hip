@@ -41,6 +41,10 @@
 namespace vt {
 
 constexpr int kWalkCap = 128;  // pending ray points per lane (device memory)
+// Intervals walked for another lane of the wave: events a lane can record
+// for one interval, and intervals a lane can have given away at once.
+constexpr int kHelpEvents = 128;
+constexpr int kHelpGifts = 16;
 
 // ---------------------------------------------------------------- sinks ----
 
@@ -177,6 +181,7 @@ __device__ __forceinline__ void put_count(int64_t *counts, int64_t ray, uint64_t
 template <int NF>
 struct SumIO {
   using Sink = SumSink<NF>;
+  static constexpr bool kHelp = true;
   const double *fields;
   double *out;
   int nout;
@@ -194,6 +199,7 @@ struct SumIO {
 
 struct SumDynIO {
   using Sink = SumSinkDyn;
+  static constexpr bool kHelp = true;
   const double *fields;
   int nf;
   double *out;
@@ -214,6 +220,7 @@ struct SumDynIO {
 template <bool kMax>
 struct ExtIO {
   using Sink = ExtSink<kMax>;
+  static constexpr bool kHelp = true;
   const double *fields;
   int nf;
   double *out;
@@ -233,6 +240,7 @@ struct ExtIO {
 
 struct VolIO {
   using Sink = VolSink;
+  static constexpr bool kHelp = false;
   const double *fields;
   double *out;
   int nout;
@@ -250,6 +258,7 @@ struct VolIO {
 
 struct CountIO {
   using Sink = CountSink;
+  static constexpr bool kHelp = true;
   int64_t *counts;
   __device__ void begin(int64_t, Sink &sk) const { sk.init(); }
   __device__ int end(int64_t ray, Sink &sk) const {
@@ -262,6 +271,7 @@ struct CountIO {
 template <class InnerIO>
 struct FillIO {
   using Sink = FillSink<typename InnerIO::Sink>;
+  static constexpr bool kHelp = false;
   InnerIO inner;
   const int64_t *offsets;
   const uint32_t *slot_id;
@@ -308,7 +318,8 @@ __device__ __forceinline__ void run_rays(const DevCloud &C, const Scratch &S, co
   VT_LANE_SETUP(PER)
   (void)lane_info;
   EngineCounters cnt;
-  run_engine<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io, st, cnt);
+  run_engine<PER, EXP, !EXP && IO::kHelp>(C, S, start, end, nrays, ray_list, defer, err, stats,
+                                          io, st, cnt);
   flush_stats(stats, cnt.nseg, cnt.nq + cnt.nslow, cnt.nslow, cnt.maxw, st.maxsp);
 }
 
@@ -573,6 +584,7 @@ struct vt_cloud {
   Stats *d_stats = nullptr;
   Scratch scratch{};
   size_t scratch_lanes = 0;
+  uint4 *help_ev = nullptr, *help_steal = nullptr;  // interval buffers (scratch_lanes lanes)
   uint32_t ray_row = 0;         // rays per image row of the coming calls (0 = no image)
   uint32_t *d_order = nullptr;  // a caller's ray order for calls of order_n rays
   size_t order_cap = 0;
@@ -591,7 +603,8 @@ struct vt_cloud {
     for (void *p : {(void *)d_nodes, (void *)d_recs, (void *)d_gen, (void *)d_fields, (void *)d_slot_id,
                     (void *)d_err, (void *)d_stats, (void *)scratch.knn_spill,
                     (void *)scratch.walk_tag, (void *)scratch.walk_s,
-                    (void *)scratch.big_id, (void *)scratch.big_d, (void *)d_defer,
+                    (void *)scratch.big_id, (void *)scratch.big_d, (void *)help_ev,
+                    (void *)help_steal, (void *)d_defer,
                     (void *)d_order, (void *)d_order_hist, (void *)d_order_grid})
       if (p) (void)hipFree(p);
     d_order_hist = nullptr;
@@ -613,6 +626,8 @@ struct vt_cloud {
     d_stats = nullptr;
     scratch = Scratch{};
     scratch_lanes = 0;
+    help_ev = nullptr;
+    help_steal = nullptr;
   }
   ~vt_cloud() { free_device(); }
 };
@@ -659,7 +674,8 @@ int grid_for(K kernel, int64_t items, int dev) {
 // Per-lane device scratch for `lanes` persistent lanes.
 int env_int(const char *name, int dflt, int lo, int hi) {
   const char *e = std::getenv(name);
-  const int b = e ? std::atoi(e) : 0;
+  if (!e || !*e) return dflt;
+  const int b = std::atoi(e);
   return (b >= lo && b <= hi) ? b : dflt;
 }
 
@@ -671,34 +687,54 @@ int env_int(const char *name, int dflt, int lo, int hi) {
 std::atomic<int> g_walk_batch{env_int("VT_WALK_BATCH", 32, 1, 64)};
 std::atomic<int> g_leaf_batch{env_int("VT_LEAF_BATCH", 32, 1, 64)};
 std::atomic<int> g_ray_parts{env_int("VT_RAY_PARTS", kParts, 1, kParts) >= kParts ? kParts : 1};
+// Lanes with nothing left to fetch walk intervals of other lanes' rays,
+// recording up to this many boundaries of one interval (0: they do not;
+// VT_HELP_EVENTS overrides it at load).
+std::atomic<int> g_help_events{env_int("VT_HELP_EVENTS", kHelpEvents, 0, kHelpEvents)};
+int help_events() { return g_help_events.load(); }
 int walk_batch() { return g_walk_batch.load(); }
 int leaf_batch() { return g_leaf_batch.load(); }
 int ray_parts() { return g_ray_parts.load(); }
 
 void ensure_scratch(vt_cloud *c, size_t lanes) {
+  if (lanes > c->scratch_lanes) {
+    for (void *p : {(void *)c->scratch.knn_spill, (void *)c->scratch.walk_tag,
+                    (void *)c->scratch.walk_s, (void *)c->scratch.big_id,
+                    (void *)c->scratch.big_d, (void *)c->help_ev, (void *)c->help_steal})
+      if (p) (void)hipFree(p);
+    c->scratch = Scratch{};
+    c->scratch_lanes = 0;
+    c->help_ev = nullptr;
+    c->help_steal = nullptr;
+    Scratch s{};
+    s.kspill = c->kspill;
+    s.wcap = kWalkCap;
+    s.lanes = (int)lanes;
+    // Every pointer is stored in c->scratch as soon as it is allocated, so a
+    // failed allocation leaves nothing unowned.
+    c->scratch.knn_spill = s.knn_spill = dev_alloc<uint4>((size_t)s.kspill * lanes);
+    c->scratch.walk_tag = s.walk_tag = dev_alloc<uint32_t>((size_t)s.wcap * lanes);
+    c->scratch.walk_s = s.walk_s = dev_alloc<double>((size_t)s.wcap * lanes);
+    const size_t waves = (lanes + 63) / 64;
+    c->scratch.big_id = s.big_id = dev_alloc<uint32_t>(waves * 4 * kMaxQueryK);
+    c->scratch.big_d = s.big_d = dev_alloc<double>(waves * 4 * kMaxQueryK);
+    c->scratch = s;
+    c->scratch_lanes = lanes;
+  }
+  // Interval buffers: allocated on the first call that walks intervals for
+  // other lanes, for every lane of the scratch.
+  const int hev = help_events();
+  if (hev > 0 && !c->help_steal) {
+    if (!c->help_ev) c->help_ev = dev_alloc<uint4>((size_t)kHelpEvents * c->scratch_lanes);
+    c->help_steal = dev_alloc<uint4>((size_t)2 * kHelpGifts * c->scratch_lanes);
+  }
+  c->scratch.help_ev = hev > 0 ? c->help_ev : nullptr;
+  c->scratch.help_steal = hev > 0 ? c->help_steal : nullptr;
+  c->scratch.help_cap = hev;
+  c->scratch.help_steal_cap = kHelpGifts;
   c->scratch.walk_batch = walk_batch();
   c->scratch.leaf_batch = leaf_batch();
   c->scratch.parts = ray_parts();
-  if (lanes <= c->scratch_lanes) return;
-  for (void *p : {(void *)c->scratch.knn_spill,
-                  (void *)c->scratch.walk_tag, (void *)c->scratch.walk_s,
-                  (void *)c->scratch.big_id, (void *)c->scratch.big_d})
-    if (p) (void)hipFree(p);
-  Scratch s{};
-  s.kspill = c->kspill;
-  s.wcap = kWalkCap;
-  s.lanes = (int)lanes;
-  s.knn_spill = dev_alloc<uint4>((size_t)s.kspill * lanes);
-  s.walk_tag = dev_alloc<uint32_t>((size_t)s.wcap * lanes);
-  s.walk_s = dev_alloc<double>((size_t)s.wcap * lanes);
-  const size_t waves = (lanes + 63) / 64;
-  s.big_id = dev_alloc<uint32_t>(waves * 4 * kMaxQueryK);
-  s.big_d = dev_alloc<double>(waves * 4 * kMaxQueryK);
-  s.walk_batch = walk_batch();
-  s.leaf_batch = leaf_batch();
-  s.parts = ray_parts();
-  c->scratch = s;
-  c->scratch_lanes = lanes;
 }
 
 void ensure_defer(vt_cloud *c, size_t nrays) {
@@ -734,6 +770,8 @@ int finish_call(vt_cloud *c, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool 
   g_stats.slow_queries += (int64_t)ss.slow_queries;
   g_stats.deferred_rays += (int64_t)ss.deferred;
   g_stats.kernel_ms += ms;
+  g_stats.intervals += (int64_t)ss.intervals;
+  g_stats.intervals_back += (int64_t)ss.intervals_back;
   if (!er.flag) return VT_OK;
   auto cell = [&](uint32_t slot) -> size_t {
     return slot < c->tree.vind.size() ? (size_t)c->tree.vind[slot] : (size_t)slot;
@@ -767,6 +805,10 @@ int finish_call(vt_cloud *c, hipStream_t st, hipEvent_t e0, hipEvent_t e1, bool 
                                  (long long)er.ray));
     case kErrFillOverflow:
       return fail(VT_ERUNTIME, fmt("Ray %lld: segment count changed between passes",
+                                   (long long)er.ray));
+    case kErrHelpWait:
+      return fail(VT_ERUNTIME, fmt("Ray %lld: internal error: a lane waited too long for "
+                                   "another lane of its wave",
                                    (long long)er.ray));
     case kErrTies:
       return fail(VT_ERUNTIME, fmt("Ray %lld: 100 or more generators equidistant from one "
@@ -2080,13 +2122,15 @@ int vt_grid_rays(const vt_view *views, size_t nviews, double *d_start, double *d
 int vt_set_tuning(const vt_tuning *t) {
   if (!t) return fail(VT_EINVAL, "null argument");
   if (t->walk_batch < 1 || t->walk_batch > 64 || t->leaf_batch < 1 || t->leaf_batch > 64 ||
-      (t->ray_parts != 1 && t->ray_parts != kParts))
+      (t->ray_parts != 1 && t->ray_parts != kParts) || t->help_events < 0 ||
+      t->help_events > kHelpEvents)
     return fail(VT_EINVAL, fmt("tuning out of range: walk_batch and leaf_batch in [1, 64], "
-                               "ray_parts 1 or %d",
-                               kParts));
+                               "ray_parts 1 or %d, help_events in [0, %d]",
+                               kParts, kHelpEvents));
   g_walk_batch.store(t->walk_batch);
   g_leaf_batch.store(t->leaf_batch);
   g_ray_parts.store(t->ray_parts);
+  g_help_events.store(t->help_events);
   return VT_OK;
 }
 
@@ -2095,6 +2139,7 @@ int vt_get_tuning(vt_tuning *t) {
   t->walk_batch = walk_batch();
   t->leaf_batch = leaf_batch();
   t->ray_parts = ray_parts();
+  t->help_events = help_events();
   return VT_OK;
 }
 

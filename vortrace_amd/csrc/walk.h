@@ -172,13 +172,23 @@ struct WalkErr {
 // S.walk_tag / S.walk_s at [depth * S.lanes + wbase]; the entry right below
 // the top is also kept in registers, so taking it off the stack does not wait
 // for memory (its replacement is read then, and needed one pop later).
+//
+// Any two adjacent points of the list are walked the same way whatever the
+// rest of the list holds (src/ray.cpp:250-424 reads only the current point,
+// the next one and the ray), so the walk of a ray can be split by intervals:
+// another lane may take the bottom pair of the pending stack (the interval
+// nearest the ray's end, steal_bottom) and walk it on its own
+// (begin_interval); its case-2 boundaries and zero-width switches, replayed
+// in order after this lane's own, give the same segments.  Entries below
+// `bot` have been given away.
 template <bool kPer>
 struct Walker {
   RayGeom r;
   uint32_t ct, nt;  // current point, and the top of the pending stack ("next")
   double sc, sn;
-  int wsp;  // pending entries below the top (entries 0 .. wsp-2 in device memory)
-  uint32_t nt2;  // entry wsp-1, when wsp > 0
+  int wsp;  // pending entries below the top (entries bot .. wsp-2 in device memory)
+  int bot;  // entries below bot were given to other lanes
+  uint32_t nt2;  // entry wsp-1, when wsp > bot
   double sn2;
   uint32_t open_cell;
   double entry_s;
@@ -198,6 +208,7 @@ struct Walker {
     sc = 0.0;
     sn = r.len;
     wsp = 0;
+    bot = 0;
     nt2 = 0;
     sn2 = 0.0;
     open_cell = 0;
@@ -212,15 +223,16 @@ struct Walker {
 
   __device__ __forceinline__ bool push_top(const Scratch &S, size_t wbase, int &max_depth,
                                            uint32_t cell, double sv) {
+    if (wsp == bot) wsp = bot = 0;  // nothing below the top: start the memory part over
     if (wsp >= S.wcap) return false;
-    if (wsp > 0) {
+    if (wsp > bot) {
       gst(S.walk_tag + (size_t)(wsp - 1) * S.lanes + wbase, nt2);
       gst(S.walk_s + (size_t)(wsp - 1) * S.lanes + wbase, sn2);
     }
     nt2 = nt;
     sn2 = sn;
     ++wsp;
-    max_depth = wsp > max_depth ? wsp : max_depth;
+    max_depth = wsp - bot > max_depth ? wsp - bot : max_depth;
     nt = cell;
     sn = sv;
     return true;
@@ -229,15 +241,52 @@ struct Walker {
   __device__ __forceinline__ bool advance(const Scratch &S, size_t wbase) {
     ct = nt;
     sc = sn;
-    if (wsp == 0) return false;
+    if (wsp == bot) return false;
     --wsp;
     nt = nt2;
     sn = sn2;
-    if (wsp > 0) {
+    if (wsp > bot) {
       nt2 = gld(S.walk_tag + (size_t)(wsp - 1) * S.lanes + wbase);
       sn2 = gld(S.walk_s + (size_t)(wsp - 1) * S.lanes + wbase);
     }
     return true;
+  }
+  // Pending pairs that can be given away: the bottom pair must lie below the
+  // two entries kept in registers.
+  __device__ __forceinline__ bool can_give() const { return wsp - bot >= 3; }
+  // Gives the bottom pair (p, q) of the pending stack away: q leaves the list,
+  // p becomes its last point.
+  __device__ __forceinline__ void steal_bottom(const Scratch &S, size_t wbase, uint32_t &pc,
+                                               double &ps, uint32_t &qc, double &qs) {
+    pc = gld(S.walk_tag + (size_t)(bot + 1) * S.lanes + wbase);
+    ps = gld(S.walk_s + (size_t)(bot + 1) * S.lanes + wbase);
+    qc = gld(S.walk_tag + (size_t)bot * S.lanes + wbase);
+    qs = gld(S.walk_s + (size_t)bot * S.lanes + wbase);
+    ++bot;
+  }
+  // Walks the interval between points (pc, ps) and (qc, qs) of the ray from
+  // s0 to e0, as the ray's own walk would from the moment that pair is the
+  // current one.
+  __device__ __forceinline__ void begin_interval(const double *s0, const double *e0, uint32_t pc,
+                                                 double ps, uint32_t qc, double qs) {
+    begin(s0, e0);
+    ct = pc;
+    sc = ps;
+    nt = qc;
+    sn = qs;
+    c0 = pc;
+    open_cell = pc;
+    state = kStSplit;
+  }
+  // A pair given away came back unwalked (its walker ran out of room): it
+  // becomes the current pair, with nothing pending below it.
+  __device__ __forceinline__ void take_back(uint32_t pc, double ps, uint32_t qc, double qs) {
+    ct = pc;
+    sc = ps;
+    nt = qc;
+    sn = qs;
+    wsp = bot = 0;
+    state = kStSplit;
   }
   // The split point was found (analytically or by bisection): bounds check,
   // clamp (src/ray.cpp:280-291), then classify.
@@ -326,8 +375,10 @@ struct Walker {
   // every entry within kTol of the nearest is found exactly, and an entry at
   // or beyond B (left as the placeholder (none, B)) passes both tests as
   // the true one does.  When 2 kTol is below the resolution of m, no bound.
-  __device__ __forceinline__ int prepare(const DevCloud &C, const Scratch &S, size_t wbase, const double *end,
-                         double (&q)[3], int &k, double &bound, WalkErr &e) {
+  template <class Sink>
+  __device__ __forceinline__ int prepare(const DevCloud &C, const Scratch &S, size_t wbase,
+                                         const double *end, double (&q)[3], int &k,
+                                         double &bound, WalkErr &e, Sink &sink) {
     bound = DBL_MAX;
     double p1[3], p2[3];
     bool have_pair = false;
@@ -337,6 +388,7 @@ struct Walker {
         if (sn - sc < 1e-14) {  // zero-width step
           const bool more = advance(S, wbase);
           open_cell = ct;
+          sink.sw(ct);
           if (!more) {
             state = kStFinal;
             return kErrNone;
@@ -511,7 +563,7 @@ struct Walker {
       }
       if (has_c || has_n) {
         // Case 2: the c/n boundary.
-        sink.seg(open_cell, entry_s, s);
+        sink.close(open_cell, entry_s, s, nt);
         open_cell = nt;
         entry_s = s;
         state = advance(S, wbase) ? kStSplit : kStFinal;

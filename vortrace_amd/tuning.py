@@ -4,7 +4,9 @@ The ray engine (vortrace_amd/csrc/engine.h) runs node steps every iteration
 and groups leaf steps and walk steps: a wave runs its leaf steps once
 ``leaf_batch`` of its lanes are at a leaf, and its walk steps once
 ``walk_batch`` lanes wait for one.  Rays are taken from ``ray_parts``
-partitions (8: one per XCD, for L2 locality; or 1).  Results never depend on
+partitions (8: one per XCD, for L2 locality; or 1).  Once a lane has no ray
+left to take, it walks an interval of a busy lane's ray of its wave, recording
+up to ``help_events`` boundaries of it (0: off).  Results never depend on
 these values, only speed.  The kd-tree's leaf size stays the reference's 10
 (src/pointcloud.cpp:204): the order of equidistant neighbours, and with it the
 crossing sequences, depends on it.
@@ -22,23 +24,25 @@ from . import _lib
 def get_tuning() -> dict:
     t = _lib.Tuning()
     _lib.check(_lib.lib().vt_get_tuning(C.byref(t)))
-    return {"walk_batch": t.walk_batch, "leaf_batch": t.leaf_batch, "ray_parts": t.ray_parts}
+    return {"walk_batch": t.walk_batch, "leaf_batch": t.leaf_batch, "ray_parts": t.ray_parts,
+            "help_events": t.help_events}
 
 
 def set_tuning(walk_batch: int | None = None, leaf_batch: int | None = None,
-               ray_parts: int | None = None) -> dict:
+               ray_parts: int | None = None, help_events: int | None = None) -> dict:
     """Sets the given parameters (process-wide); returns the new settings."""
     cur = get_tuning()
     t = _lib.Tuning(int(walk_batch if walk_batch is not None else cur["walk_batch"]),
                     int(leaf_batch if leaf_batch is not None else cur["leaf_batch"]),
-                    int(ray_parts if ray_parts is not None else cur["ray_parts"]))
+                    int(ray_parts if ray_parts is not None else cur["ray_parts"]),
+                    int(help_events if help_events is not None else cur["help_events"]))
     _lib.check(_lib.lib().vt_set_tuning(C.byref(t)))
     return get_tuning()
 
 
 def autotune(pc, pos_start, pos_end, *, reduction: str = "integrate", stride: int = 4,
              walk_batches=(16, 24, 32, 48), leaf_batches=(16, 24, 32, 40),
-             ray_parts=(1, 8), repeats: int = 2) -> dict:
+             ray_parts=(1, 8), help_events=(128,), repeats: int = 2) -> dict:
     """Times every combination on every `stride`-th ray of (pos_start,
     pos_end) through ``pc`` (a ProjectionCloud) and keeps the fastest.
     Returns {"best": settings, "kernel_ms": best time, "results": [...]}."""
@@ -48,14 +52,14 @@ def autotune(pc, pos_start, pos_end, *, reduction: str = "integrate", stride: in
     s, e = s.to(dev).contiguous(), e.to(dev).contiguous()
     pc.projection_tensor(s, e, reduction=reduction)  # warm-up
     results = []
-    for wb, lb, rp in itertools.product(walk_batches, leaf_batches, ray_parts):
-        set_tuning(wb, lb, rp)
+    for wb, lb, rp, he in itertools.product(walk_batches, leaf_batches, ray_parts, help_events):
+        set_tuning(wb, lb, rp, he)
         ms = []
         for _ in range(repeats):
             pc.projection_tensor(s, e, reduction=reduction)
             ms.append(_lib.last_call_stats().kernel_ms)
         results.append({"walk_batch": wb, "leaf_batch": lb, "ray_parts": rp,
-                        "kernel_ms": min(ms)})
+                        "help_events": he, "kernel_ms": min(ms)})
     best = min(results, key=lambda r: r["kernel_ms"])
-    set_tuning(best["walk_batch"], best["leaf_batch"], best["ray_parts"])
+    set_tuning(best["walk_batch"], best["leaf_batch"], best["ray_parts"], best["help_events"])
     return {"best": get_tuning(), "kernel_ms": best["kernel_ms"], "results": results}
