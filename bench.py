@@ -447,7 +447,7 @@ def main():
                     help="fraction of the rays timed on the CPU reference (default: a "
                          "quarter, or about 2.5e8 crossings if fewer)")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="multi-view workload: keep the current launch parameters")
+                    help="keep the library's default launch parameters")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ray-order", choices=("tiles", "rows"), default="tiles",
                     help="tiles: the kernels take image rays in 8x8-pixel tiles when every "
@@ -534,12 +534,19 @@ def main():
     pc.set_ray_rows(ray_row)
     log(f"[bench] ray order: {'8x8 tiles of %d-ray rows' % ray_row if ray_row else 'index'}")
 
+    # Launch parameters for this workload (speed only, never a result): a
+    # small sweep before the timed steps, on every ray of the step (every
+    # 16th of the multi-view launch, every 4th above 2M rays).  The volume
+    # kernels do not walk other lanes' intervals, so that knob is fixed there.
     tuning = None
-    if kind == "views" and not args.no_autotune:
-        tuning = vt.tuning.autotune(pc, d_start, d_end, reduction=mode, stride=16,
-                                    walk_batches=(24, 32, 48), leaf_batches=(16, 24, 32),
-                                    ray_parts=(8,), repeats=1)
-        log(f"[bench] autotune on every 16th ray: {tuning['best']} "
+    if not args.no_autotune:
+        stride = 16 if kind == "views" else (4 if R > 2_000_000 else 1)
+        helps = (128,) if mode == "volume" else (0, 128)
+        tuning = vt.tuning.autotune(pc, d_start, d_end, reduction=mode, stride=stride,
+                                    walk_batches=(24, 32, 48), leaf_batches=(24, 32),
+                                    ray_parts=(8,), help_events=helps, repeats=1)
+        tuning["stride"] = stride
+        log(f"[bench] autotune on every {stride} ray(s): {tuning['best']} "
             f"({tuning['kernel_ms']:.2f} ms)")
 
     def forward():

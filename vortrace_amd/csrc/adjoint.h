@@ -79,7 +79,7 @@ constexpr int kRecChunk = 32;
 // values: the backward pass writes both to the same place.
 template <class InnerIO>
 struct RecordIO {
-  static constexpr bool kHelp = false;
+  static constexpr bool kHelp = InnerIO::kHelp;
   struct Sink {
     typename InnerIO::Sink in;
     RecPool pool;
@@ -151,14 +151,16 @@ __global__ void k_set_fields(const double *src, const uint32_t *slot_id, uint32_
 }
 
 // Sum: contribution of segment j of ray r to field f is g[r, f] * ds_j.
+// One wave per ray, its lanes over the ray's segments (coalesced).
 __global__ void k_contrib_sum(int64_t nrays, const int64_t *offsets, const double *ds,
                               const double *g, int nf, double *contrib) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = (int)(threadIdx.x & 63);
   if (r >= nrays) return;
   const int64_t a = gld(offsets + r), b = gld(offsets + r + 1);
   for (int f = 0; f < nf; ++f) {
     const double gf = gld(g + r * nf + f);
-    for (int64_t j = a; j < b; ++j) gst(contrib + j * nf + f, gf * gld(ds + j));
+    for (int64_t j = a + lane; j < b; j += 64) gst(contrib + j * nf + f, gf * gld(ds + j));
   }
 }
 
@@ -223,6 +225,18 @@ __global__ void k_reduce_cells(uint32_t n, int nf, const uint32_t *start, const 
     for (uint32_t j = a; j < b; ++j) acc += gld(contrib + (size_t)gld(perm + j) * nf + f);
     gst(grad + i * nf + f, acc);
   }
+}
+
+// The same sums for one field when the contributions themselves were sorted
+// by slot (stably, so each cell's run is in ray-major order): contiguous reads.
+__global__ void k_reduce_sorted(uint32_t n, const uint32_t *start, const uint32_t *end,
+                                const double *sorted, const uint32_t *slot_id, double *grad) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t a = gld(start + s), b = gld(end + s);
+  double acc = 0.0;
+  for (uint32_t j = a; j < b; ++j) acc += gld(sorted + j);
+  gst(grad + gld(slot_id + s), acc);
 }
 
 // Marks a ray order that is not a permutation of [0, n): an entry out of

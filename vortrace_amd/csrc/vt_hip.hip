@@ -342,12 +342,12 @@ __device__ __forceinline__ void run_io(const DevCloud &C, const Scratch &S, cons
   }
 }
 
+// At most 256 VGPRs (two waves per SIMD): the recording kernels with the
+// helping code would otherwise take a few more and run one wave per SIMD.
 template <int MODE, int NF, bool PER, bool REC = false>
-__global__ __launch_bounds__(kBlock) void k_project(DevCloud C, Scratch S, const double *start,
-                                                    const double *end, int64_t nrays,
-                                                    uint32_t *defer, double *out, int nout,
-                                                    int64_t *counts, RecPool pool, ErrRec *err,
-                                                    Stats *stats) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_project(
+    DevCloud C, Scratch S, const double *start, const double *end, int64_t nrays, uint32_t *defer,
+    double *out, int nout, int64_t *counts, RecPool pool, ErrRec *err, Stats *stats) {
   if constexpr (MODE == VT_SUM && NF > 0) {
     run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
                             SumIO<NF>{C.fields, out, nout, counts}, pool);
@@ -1051,8 +1051,9 @@ int bits_for(size_t n) {
 }
 
 // Steps after the segment lists of an adjoint call: contributions per
-// segment, the stable sort of segment indices by slot, and the per-cell sums
-// in ray-major order.  Returns the GPU time of these steps in ms.
+// segment, a stable sort by slot (of the contributions themselves when there
+// is one per segment, else of the segment indices), and the per-cell sums in
+// ray-major order.  Returns the GPU time of these steps in ms.
 float adjoint_tail(vt_cloud *c, int mode, int64_t n, const int64_t *offsets,
                    const uint32_t *slot, const double *ds, int64_t nseg,
                    const double *d_grad_out, double *d_grad_fields, DevBufs &B, hipStream_t st) {
@@ -1064,19 +1065,46 @@ float adjoint_tail(vt_cloud *c, int mode, int64_t n, const int64_t *offsets,
   const int threads = 256;
   const int rblocks = (int)((n + threads - 1) / threads);
   if (mode == VT_SUM)
-    k_contrib_sum<<<rblocks, threads, 0, st>>>(n, offsets, ds, d_grad_out, nf, contrib);
+    k_contrib_sum<<<(int)((n * 64 + threads - 1) / threads), threads, 0, st>>>(
+        n, offsets, ds, d_grad_out, nf, contrib);
   else
     k_contrib_vol<<<rblocks, threads, 0, st>>>(n, offsets, slot, ds, c->d_fields, d_grad_out,
                                                contrib);
   HIP_OK(hipGetLastError());
+  const int sblocks = (int)((nseg + threads - 1) / threads);
+  const int cblocks = (int)((c->n + threads - 1) / threads);
+  const int nb = bits_for(c->n);
+  if (ncf == 1) {
+    // One value per segment: sort the contributions themselves by slot.
+    uint32_t *keys = B.get<uint32_t>(nseg);
+    double *sorted = B.get<double>(nseg);
+    size_t tmp = 0;
+    HIP_OK(rocprim::radix_sort_pairs(nullptr, tmp, slot, keys, contrib, sorted, (size_t)nseg, 0,
+                                     nb, st));
+    void *t = B.get<char>(tmp);
+    HIP_OK(rocprim::radix_sort_pairs(t, tmp, slot, keys, contrib, sorted, (size_t)nseg, 0, nb,
+                                     st));
+    uint32_t *rs = B.get<uint32_t>(c->n);
+    uint32_t *re = B.get<uint32_t>(c->n);
+    HIP_OK(hipMemsetAsync(rs, 0, c->n * sizeof(uint32_t), st));
+    HIP_OK(hipMemsetAsync(re, 0, c->n * sizeof(uint32_t), st));
+    k_run_bounds<<<sblocks, threads, 0, st>>>(nseg, keys, rs, re);
+    HIP_OK(hipGetLastError());
+    k_reduce_sorted<<<cblocks, threads, 0, st>>>((uint32_t)c->n, rs, re, sorted, c->d_slot_id,
+                                                  d_grad_fields);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord(tm.e1, st));
+    HIP_OK(hipStreamSynchronize(st));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, tm.e0, tm.e1));
+    return ms;
+  }
   uint32_t *idx = B.get<uint32_t>(nseg);
   uint32_t *keys = B.get<uint32_t>(nseg);
   uint32_t *perm = B.get<uint32_t>(nseg);
-  const int sblocks = (int)((nseg + threads - 1) / threads);
   k_iota<<<sblocks, threads, 0, st>>>(nseg, idx);
   HIP_OK(hipGetLastError());
   {
-    const int nb = bits_for(c->n);
     size_t tmp = 0;
     HIP_OK(rocprim::radix_sort_pairs(nullptr, tmp, slot, keys, idx, perm, (size_t)nseg, 0, nb,
                                      st));
@@ -1089,7 +1117,6 @@ float adjoint_tail(vt_cloud *c, int mode, int64_t n, const int64_t *offsets,
   HIP_OK(hipMemsetAsync(re, 0, c->n * sizeof(uint32_t), st));
   k_run_bounds<<<sblocks, threads, 0, st>>>(nseg, keys, rs, re);
   HIP_OK(hipGetLastError());
-  const int cblocks = (int)((c->n + threads - 1) / threads);
   k_reduce_cells<<<cblocks, threads, 0, st>>>((uint32_t)c->n, ncf, rs, re, perm, contrib,
                                                c->d_slot_id, d_grad_fields);
   HIP_OK(hipGetLastError());

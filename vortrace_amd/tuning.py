@@ -40,12 +40,25 @@ def set_tuning(walk_batch: int | None = None, leaf_batch: int | None = None,
     return get_tuning()
 
 
+_CACHE: dict = {}
+
+
 def autotune(pc, pos_start, pos_end, *, reduction: str = "integrate", stride: int = 4,
              walk_batches=(16, 24, 32, 48), leaf_batches=(16, 24, 32, 40),
-             ray_parts=(1, 8), help_events=(128,), repeats: int = 2) -> dict:
+             ray_parts=(1, 8), help_events=(128,), repeats: int = 2,
+             cache: bool = True) -> dict:
     """Times every combination on every `stride`-th ray of (pos_start,
     pos_end) through ``pc`` (a ProjectionCloud) and keeps the fastest.
-    Returns {"best": settings, "kernel_ms": best time, "results": [...]}."""
+    Returns {"best": settings, "kernel_ms": best time, "results": [...],
+    "cached": bool}.  With `cache`, a second call for the same cloud, ray
+    count, reduction and search space applies the stored best settings
+    without timing again."""
+    key = (id(pc._cloud), len(pos_start), str(reduction), int(stride), tuple(walk_batches),
+           tuple(leaf_batches), tuple(ray_parts), tuple(help_events))
+    if cache and key in _CACHE:
+        hit = _CACHE[key]
+        set_tuning(**hit["best"])
+        return dict(hit, cached=True)
     dev = pc._cloud._device
     s = torch.as_tensor(pos_start, dtype=torch.float64).reshape(-1, 3)[::stride]
     e = torch.as_tensor(pos_end, dtype=torch.float64).reshape(-1, 3)[::stride]
@@ -62,4 +75,7 @@ def autotune(pc, pos_start, pos_end, *, reduction: str = "integrate", stride: in
                         "help_events": he, "kernel_ms": min(ms)})
     best = min(results, key=lambda r: r["kernel_ms"])
     set_tuning(best["walk_batch"], best["leaf_batch"], best["ray_parts"], best["help_events"])
-    return {"best": get_tuning(), "kernel_ms": best["kernel_ms"], "results": results}
+    out = {"best": get_tuning(), "kernel_ms": best["kernel_ms"], "results": results}
+    if cache:
+        _CACHE[key] = out
+    return dict(out, cached=False)
