@@ -322,8 +322,12 @@ def fwd_adjoint(pc, fields, d_start, d_end, args, C, R, mode="sum"):
         split["backward_ms"].append(1e3 * (time.perf_counter() - t1))
         return f.grad, pc._forward_stats, pc._adjoint_stats
 
+    first = None  # the first pass over these rays (record pool from a sample)
     for i in range(args.warmup):
         step()
+        if first is None:
+            first = {"ms": split["forward_ms"][0] + split["backward_ms"][0],
+                     "adjoint_path": pc._adjoint_path}
         log(f"[bench] fwd+adjoint warmup {i + 1}/{args.warmup} done")
     split["forward_ms"].clear()
     split["backward_ms"].clear()
@@ -345,7 +349,7 @@ def fwd_adjoint(pc, fields, d_start, d_end, args, C, R, mode="sum"):
     rep = {"ms_per_step": ms, "crossings_per_s": C / (ms * 1e-3), "rays_per_s": R / (ms * 1e-3),
            "forward_kernel_ms": float(np.mean(fms)),
            "adjoint_kernel_ms": float(np.mean(kms)), "forward_ms": fwd_ms,
-           "backward_ms": bwd_ms, "cells_with_gradient": nz,
+           "backward_ms": bwd_ms, "cells_with_gradient": nz, "first_step": first,
            "step": "L = sum (proj(fields) - target)^2 and dL/dfields by torch.autograd "
                    "(forward kernel recording its segments + adjoint kernels over the "
                    "records), FP64"}

@@ -240,3 +240,29 @@ def test_autograd_records_after_first_pass():
         assert pc._adjoint_path == ("walk" if len(grads) == 1 else "recorded")
     assert torch.equal(grads[0], grads[1]) and torch.equal(grads[1], grads[2])
     assert int(torch.count_nonzero(grads[0])) > 0
+
+
+def test_autograd_first_pass_sampled_pool(monkeypatch):
+    """The first pass over a ray set sizes its record pool from a sample of
+    the rays and records; a pool that falls short leaves the backward pass to
+    walk the rays.  The gradient has the same bits either way."""
+    _say("autograd first pass, sampled pool")
+    pc, f_np, ds_, de_, _ = _recorded_case("sum")
+    monkeypatch.setattr(vt.autograd, "_SAMPLE_RAYS", 16)  # every 6th of 1600 rays
+    f0 = torch.from_numpy(f_np).to(ds_.device)
+    w = torch.from_numpy(np.random.default_rng(36).standard_normal((ds_.shape[0], 2))).to(
+        ds_.device)
+
+    def first_pass():
+        vt.autograd._ray_states(pc).clear()
+        f = f0.clone().requires_grad_(True)
+        (vt.autograd.project(pc, f, ds_, de_, "sum") * w).sum().backward()
+        return f.grad, pc._adjoint_path
+
+    g1, path1 = first_pass()
+    assert path1 == "recorded"
+    # a pool far too small for the sampled estimate: overflow, then a walk
+    monkeypatch.setattr(vt.autograd, "_pool_capacity", lambda est, lanes: max(1, est // 100))
+    g2, path2 = first_pass()
+    assert path2 == "walk"
+    assert torch.equal(g1, g2) and int(torch.count_nonzero(g1)) > 0

@@ -16,7 +16,8 @@ pool while it projects, and the backward pass reorders those records instead
 of walking the rays again (``vt_project_recorded`` / ``vt_adjoint_recorded``).
 The pool is sized from the segment count of the previous forward pass over
 the same rays (the same ray tensors or arrays, unchanged; a few ray sets are
-remembered per cloud); on the first pass over a ray set, or when the pool
+remembered per cloud), or on the first pass over a ray set from a sample
+(every k-th ray projected on its own, about 1/64 of the work); when the pool
 overflows, the backward pass walks the rays for their segments
 (``vt_adjoint``).  Both give the same bits.
 
@@ -52,6 +53,25 @@ _REC_CHUNK = 32
 
 def _pool_capacity(estimate: int, lanes: int) -> int:
     return int(estimate * _POOL_GROWTH) + 2 * lanes * _REC_CHUNK
+
+
+# First pass over a ray set: segments estimated from every k-th ray (at least
+# _SAMPLE_RAYS of them), scaled up with headroom for the sampling error.
+_SAMPLE_RAYS = 4096
+_SAMPLE_HEADROOM = 1.25
+
+
+def _sampled_estimate(cloud, mode, pos_start, pos_end, n: int) -> int:
+    """Segments of all n rays, estimated by projecting every k-th ray."""
+    k = max(1, n // (16 * _SAMPLE_RAYS))
+    if k == 1:
+        return -1  # few rays: no estimate, the first pass walks twice
+    s = torch.as_tensor(pos_start).reshape(-1, 3)[::k]
+    e = torch.as_tensor(pos_end).reshape(-1, 3)[::k]
+    p = Projection(s, e)
+    c = torch.empty(p.ngrid, dtype=torch.int64, device=cloud._device)
+    p.project_tensor(cloud, mode, counts=c)
+    return int(float(c.sum().item()) * (n / p.ngrid) * _SAMPLE_HEADROOM)
 
 
 # Rays per block of the longest-first order: 2^15 rays of an image are a band
@@ -203,14 +223,20 @@ class ProjectFields(torch.autograd.Function):
         if ordered:
             cloud.order_rays(counts=prev, row=cloud.get_ray_rows())
         try:
-            if ctx.needs_input_grad[0] and state is not None:
+            estimate = -1
+            if ctx.needs_input_grad[0]:
+                estimate = (state.estimate if state is not None else
+                            _sampled_estimate(cloud, mode, pos_start, pos_end, proj.ngrid))
+            if estimate >= 0:
                 lanes = cloud.lanes(mode, recorded=True)
-                records = torch.empty((_pool_capacity(state.estimate, lanes), SEGREC_BYTES),
+                records = torch.empty((_pool_capacity(estimate, lanes), SEGREC_BYTES),
                                       dtype=torch.uint8, device=dev)
                 out, n_records = proj.project_recorded_tensor(cloud, mode, counts, records)
                 if n_records < 0:
                     records = None  # pool too small: the backward walks the rays
-                    if not getattr(pc, "_pool_warned", False):
+                    # (a sampled estimate may fall short; a previous pass's count
+                    # over the same rays should not)
+                    if state is not None and not getattr(pc, "_pool_warned", False):
                         pc._pool_warned = True
                         warnings.warn("vortrace_amd.autograd: the segment record pool "
                                       "overflowed; the backward pass walks the rays again "
