@@ -38,7 +38,6 @@ struct SegSink {
 
 struct SegIO {
   using Sink = SegSink;
-  static constexpr bool kHelp = false;
   const int64_t *offsets;
   uint32_t *slot;
   double *ds;
@@ -79,7 +78,6 @@ constexpr int kRecChunk = 32;
 // values: the backward pass writes both to the same place.
 template <class InnerIO>
 struct RecordIO {
-  static constexpr bool kHelp = InnerIO::kHelp;
   struct Sink {
     typename InnerIO::Sink in;
     RecPool pool;

@@ -181,7 +181,6 @@ __device__ __forceinline__ void put_count(int64_t *counts, int64_t ray, uint64_t
 template <int NF>
 struct SumIO {
   using Sink = SumSink<NF>;
-  static constexpr bool kHelp = true;
   const double *fields;
   double *out;
   int nout;
@@ -199,7 +198,6 @@ struct SumIO {
 
 struct SumDynIO {
   using Sink = SumSinkDyn;
-  static constexpr bool kHelp = true;
   const double *fields;
   int nf;
   double *out;
@@ -220,7 +218,6 @@ struct SumDynIO {
 template <bool kMax>
 struct ExtIO {
   using Sink = ExtSink<kMax>;
-  static constexpr bool kHelp = true;
   const double *fields;
   int nf;
   double *out;
@@ -240,7 +237,6 @@ struct ExtIO {
 
 struct VolIO {
   using Sink = VolSink;
-  static constexpr bool kHelp = false;
   const double *fields;
   double *out;
   int nout;
@@ -258,7 +254,6 @@ struct VolIO {
 
 struct CountIO {
   using Sink = CountSink;
-  static constexpr bool kHelp = true;
   int64_t *counts;
   __device__ void begin(int64_t, Sink &sk) const { sk.init(); }
   __device__ int end(int64_t ray, Sink &sk) const {
@@ -271,7 +266,6 @@ struct CountIO {
 template <class InnerIO>
 struct FillIO {
   using Sink = FillSink<typename InnerIO::Sink>;
-  static constexpr bool kHelp = false;
   InnerIO inner;
   const int64_t *offsets;
   const uint32_t *slot_id;
@@ -310,7 +304,11 @@ __device__ __forceinline__ void flush_stats(Stats *stats, uint64_t nseg, uint64_
   const LaneInfo lane_info{tid, gtid, gtid >> 6, (int)(tid & 63)};              \
   KStack st = make_stack(S, lane_info);
 
-template <bool PER, bool EXP, class IO>
+// With HELP (fast kernels only), idle lanes walk intervals of busy lanes'
+// rays.  Each projection and count kernel is built with and without it: the
+// helping code costs every iteration a little, and it pays only where lanes
+// run out of rays long before the launch ends (help_pays).
+template <bool PER, bool EXP, bool HELP = false, class IO>
 __device__ __forceinline__ void run_rays(const DevCloud &C, const Scratch &S, const double *start,
                                          const double *end, int64_t nrays,
                                          const uint32_t *ray_list, uint32_t *defer, ErrRec *err,
@@ -318,8 +316,8 @@ __device__ __forceinline__ void run_rays(const DevCloud &C, const Scratch &S, co
   VT_LANE_SETUP(PER)
   (void)lane_info;
   EngineCounters cnt;
-  run_engine<PER, EXP, !EXP && IO::kHelp>(C, S, start, end, nrays, ray_list, defer, err, stats,
-                                          io, st, cnt);
+  run_engine<PER, EXP, !EXP && HELP>(C, S, start, end, nrays, ray_list, defer, err, stats, io,
+                                      st, cnt);
   flush_stats(stats, cnt.nseg, cnt.nq + cnt.nslow, cnt.nslow, cnt.maxw, st.maxsp);
 }
 
@@ -328,38 +326,39 @@ __device__ __forceinline__ void run_rays(const DevCloud &C, const Scratch &S, co
 // classification to `defer`; the EXP = true variants handle every ray.
 
 // With REC every segment is also recorded to `pool` (differentiable forward).
-template <bool REC, bool PER, bool EXP, class IO>
+template <bool REC, bool PER, bool EXP, bool HELP = false, class IO>
 __device__ __forceinline__ void run_io(const DevCloud &C, const Scratch &S, const double *start,
                                        const double *end, int64_t nrays,
                                        const uint32_t *ray_list, uint32_t *defer, ErrRec *err,
                                        Stats *stats, const IO &io, const RecPool &pool) {
   if constexpr (REC) {
     RecordIO<IO> rio{io, pool};
-    run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, rio);
+    run_rays<PER, EXP, HELP>(C, S, start, end, nrays, ray_list, defer, err, stats, rio);
   } else {
     IO cio = io;
-    run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, cio);
+    run_rays<PER, EXP, HELP>(C, S, start, end, nrays, ray_list, defer, err, stats, cio);
   }
 }
 
 // At most 256 VGPRs (two waves per SIMD): the recording kernels with the
 // helping code would otherwise take a few more and run one wave per SIMD.
-template <int MODE, int NF, bool PER, bool REC = false>
+template <int MODE, int NF, bool PER, bool REC = false, bool HELP = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_project(
     DevCloud C, Scratch S, const double *start, const double *end, int64_t nrays, uint32_t *defer,
     double *out, int nout, int64_t *counts, RecPool pool, ErrRec *err, Stats *stats) {
   if constexpr (MODE == VT_SUM && NF > 0) {
-    run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
-                            SumIO<NF>{C.fields, out, nout, counts}, pool);
+    run_io<REC, PER, false, HELP>(C, S, start, end, nrays, nullptr, defer, err, stats,
+                                  SumIO<NF>{C.fields, out, nout, counts}, pool);
   } else if constexpr (MODE == VT_SUM) {
-    run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
-                            SumDynIO{C.fields, (int)C.nf, out, nout, counts}, pool);
+    run_io<REC, PER, false, HELP>(C, S, start, end, nrays, nullptr, defer, err, stats,
+                                  SumDynIO{C.fields, (int)C.nf, out, nout, counts}, pool);
   } else if constexpr (MODE == VT_MAX || MODE == VT_MIN) {
-    run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
-                            ExtIO<MODE == VT_MAX>{C.fields, (int)C.nf, out, nout, counts}, pool);
+    run_io<REC, PER, false, HELP>(
+        C, S, start, end, nrays, nullptr, defer, err, stats,
+        ExtIO<MODE == VT_MAX>{C.fields, (int)C.nf, out, nout, counts}, pool);
   } else {
-    run_io<REC, PER, false>(C, S, start, end, nrays, nullptr, defer, err, stats,
-                            VolIO{C.fields, out, nout, counts}, pool);
+    run_io<REC, PER, false, HELP>(C, S, start, end, nrays, nullptr, defer, err, stats,
+                                  VolIO{C.fields, out, nout, counts}, pool);
   }
 }
 
@@ -383,13 +382,13 @@ __global__ __launch_bounds__(kBlock) void k_project_x(DevCloud C, Scratch S,
   }
 }
 
-template <bool PER, bool EXP>
+template <bool PER, bool EXP, bool HELP = false>
 __global__ __launch_bounds__(kBlock) void k_count(DevCloud C, Scratch S, const double *start,
                                                   const double *end, int64_t nrays,
                                                   const uint32_t *ray_list, uint32_t *defer,
                                                   int64_t *counts, ErrRec *err, Stats *stats) {
   CountIO io{counts};
-  run_rays<PER, EXP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
+  run_rays<PER, EXP, HELP>(C, S, start, end, nrays, ray_list, defer, err, stats, io);
 }
 
 template <int MODE, bool PER, bool EXP>
@@ -879,6 +878,16 @@ int two_pass(vt_cloud *c, hipStream_t st, size_t nrays, Fast &&fast, Slow &&slow
   return finish_call(c, st, tm.e0, tm.e1, true);
 }
 
+// Whether a launch of n rays on `grid` workgroups uses the helping variant of
+// its kernel: at most 4 rays per lane (2 for volume, whose helping variant
+// spills registers).  Measured at 2M clustered cells: 2 rays per lane 46.8 ->
+// 41.5 ms with helping; 8 per lane (volume) 138.0 -> 139.5 ms; 32 per lane
+// +6 %, 112 per lane +6 %; 1/8 ray per lane (100k uniform) 8.4 -> 3.6 ms.
+bool help_pays(int mode, int64_t n, int grid) {
+  const int64_t lanes = (int64_t)grid * kBlock;
+  return n <= (mode == VT_VOLUME ? 2 : 4) * lanes;
+}
+
 // The ray engine's launches: a full persistent grid, and when there are
 // fewer rays than lanes, only the first fetch_lanes lanes of every wave take
 // rays, so the rays spread over all CUs instead of filling a few waves.
@@ -903,8 +912,13 @@ int project_p(vt_cloud *c, const double *s, const double *e, int64_t n, double *
   return two_pass(
       c, st, (size_t)n,
       [&](Timer &tm) {
-        launch(c, k_project<MODE, NF, PER, REC>, n, st, tm, s, e, n, c->d_defer, out, nout,
-               counts, pool, c->d_err, c->d_stats);
+        if (help_pays(MODE, n, grid_for(k_project<MODE, NF, PER, REC>, INT64_MAX / 2,
+                                        c->device)))
+          launch(c, k_project<MODE, NF, PER, REC, true>, n, st, tm, s, e, n, c->d_defer, out,
+                 nout, counts, pool, c->d_err, c->d_stats);
+        else
+          launch(c, k_project<MODE, NF, PER, REC>, n, st, tm, s, e, n, c->d_defer, out, nout,
+                 counts, pool, c->d_err, c->d_stats);
       },
       [&](Timer &tm, int64_t nd) {
         launch(c, k_project_x<MODE, PER, REC>, nd, st, tm, s, e, nd,
@@ -1740,8 +1754,13 @@ int vt_project_count(const vt_cloud *cc, const double *d_start, const double *d_
                    (uint32_t *)nullptr, d_counts, c->d_err, c->d_stats);
           });
     };
-    return c->periodic ? go(k_count<true, false>, k_count<true, true>)
-                       : go(k_count<false, false>, k_count<false, true>);
+    const bool help = help_pays(VT_SUM, n, grid_for(k_count<false, false>, INT64_MAX / 2,
+                                                    c->device));
+    if (c->periodic)
+      return help ? go(k_count<true, false, true>, k_count<true, true>)
+                  : go(k_count<true, false>, k_count<true, true>);
+    return help ? go(k_count<false, false, true>, k_count<false, true>)
+                : go(k_count<false, false>, k_count<false, true>);
   });
 }
 
