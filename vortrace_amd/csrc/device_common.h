@@ -151,6 +151,20 @@ __device__ __forceinline__ void gst(uint4 *p, uint4 e) {
   *(__attribute__((address_space(1))) v4u32 *)(p) = v;
 }
 
+// Non-temporal stores, for data the kernel writes and never reads back: a
+// partly written line is not fetched into the L2 first.
+__device__ __forceinline__ void gst_nt(uint4 *p, uint4 e) {
+  v4u32 v;
+  v.x = e.x;
+  v.y = e.y;
+  v.z = e.z;
+  v.w = e.w;
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) v4u32 *)(p));
+}
+__device__ __forceinline__ void gst_nt(double *p, double v) {
+  __builtin_nontemporal_store(v, (__attribute__((address_space(1))) double *)(p));
+}
+
 // a, b or c for f = 0, 1 or 2 (by value).
 __device__ __forceinline__ double sel3(double a, double b, double c, uint32_t f) {
   return f == 0 ? a : (f == 1 ? b : c);
