@@ -114,6 +114,7 @@ struct Scratch {
   uint4 *help_ev;      // [help_cap][T] events of the interval this lane walks
   uint4 *help_steal;   // [2 * help_steal_cap][T] intervals this lane gave away
   int help_cap, help_steal_cap;
+  int help_depth;      // pending entries below the top a lane needs to give one pair away
 };
 
 constexpr int kMaxQueryK = 128;

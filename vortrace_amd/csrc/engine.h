@@ -745,7 +745,8 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
       if (kHelp && idle) {
         // A lane that is searching, or has just prepared its next search,
         // can give: a search reads only the top pair of the pending list.
-        const bool can = (ph == kPhSearch || query) && role != kRoleIdle && W.can_give() &&
+        const bool can = (ph == kPhSearch || query) && role != kRoleIdle &&
+                         W.can_give(S.help_depth) &&
                          nsteal < S.help_steal_cap && W.wsp + 4 < S.wcap;
         const uint64_t donors = __ballot(can);
         if (donors) {

@@ -251,17 +251,32 @@ struct Walker {
     }
     return true;
   }
-  // Pending pairs that can be given away: the bottom pair must lie below the
-  // two entries kept in registers.
-  __device__ __forceinline__ bool can_give() const { return wsp - bot >= 3; }
-  // Gives the bottom pair (p, q) of the pending stack away: q leaves the list,
-  // p becomes its last point.
+  // Whether a pending pair below the top can be given away: at least
+  // `min_depth` (>= 1) entries below the top.
+  __device__ __forceinline__ bool can_give(int min_depth) const { return wsp - bot >= min_depth; }
+  // Gives the bottom pair (p, q) of the pending list away: q leaves the list,
+  // p becomes its last point.  q is entry bot (in registers as nt2 when it is
+  // entry wsp-1); p is entry bot + 1 (nt2 when that is entry wsp-1, the top
+  // itself when bot + 1 == wsp).
   __device__ __forceinline__ void steal_bottom(const Scratch &S, size_t wbase, uint32_t &pc,
                                                double &ps, uint32_t &qc, double &qs) {
-    pc = gld(S.walk_tag + (size_t)(bot + 1) * S.lanes + wbase);
-    ps = gld(S.walk_s + (size_t)(bot + 1) * S.lanes + wbase);
-    qc = gld(S.walk_tag + (size_t)bot * S.lanes + wbase);
-    qs = gld(S.walk_s + (size_t)bot * S.lanes + wbase);
+    const int d = wsp - bot;
+    if (d == 1) {
+      qc = nt2;
+      qs = sn2;
+      pc = nt;
+      ps = sn;
+    } else {
+      qc = gld(S.walk_tag + (size_t)bot * S.lanes + wbase);
+      qs = gld(S.walk_s + (size_t)bot * S.lanes + wbase);
+      if (d == 2) {
+        pc = nt2;
+        ps = sn2;
+      } else {
+        pc = gld(S.walk_tag + (size_t)(bot + 1) * S.lanes + wbase);
+        ps = gld(S.walk_s + (size_t)(bot + 1) * S.lanes + wbase);
+      }
+    }
     ++bot;
   }
   // Walks the interval between points (pc, ps) and (qc, qs) of the ray from
