@@ -17,7 +17,7 @@ of walking the rays again (``vt_project_recorded`` / ``vt_adjoint_recorded``).
 The pool is sized from the segment count of the previous forward pass over
 the same rays (the same ray tensors or arrays, unchanged; a few ray sets are
 remembered per cloud), or on the first pass over a ray set from a sample
-(every k-th ray projected on its own, about 1/64 of the work); when the pool
+(every k-th ray, about 65,536 of them, projected on its own); when the pool
 overflows, the backward pass walks the rays for their segments
 (``vt_adjoint``).  Both give the same bits.
 
