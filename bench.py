@@ -369,7 +369,9 @@ def fd_check(pc, fields, grad, target, d_start, d_end, mode, ncheck=64, seed=56)
     |p+ + p- - 2t| / 2h, n = the ray's segments), must be below
     1e-7 max(|fd|, |g|) for a cell to be checked; cells whose influence on
     the image is below that resolution are drawn again.  Checked cells must
-    satisfy |fd - g| <= 1e-6 max(|fd|, |g|)."""
+    satisfy |fd - g| <= 1e-6 max(|fd|, |g|).  The cells drawn again are
+    checked too, against what their resolution allows:
+    |fd - g| <= E + 1e-6 max(|fd|, |g|)."""
     import torch
     import vortrace_amd as vt
     dev = d_start.device
@@ -401,6 +403,7 @@ def fd_check(pc, fields, grad, target, d_start, d_end, mode, ncheck=64, seed=56)
     tgt = target.reshape(target.shape[0], -1)
     eps = float(np.finfo(np.float64).eps)
     worst, strict, ok, nrays, checked, unresolved = 0.0, 0, True, [], 0, 0
+    low_ok, low_worst = 0, 0.0  # cells below the resolution: |fd - g| / (E + 1e-6 scale)
     for c, ch, v, h in picks:
         if checked == ncheck:
             break
@@ -424,18 +427,24 @@ def fd_check(pc, fields, grad, target, d_start, d_end, mode, ncheck=64, seed=56)
         scale = max(abs(fd), abs(gv))
         if not rnd <= 1e-7 * scale:
             unresolved += 1
+            r = abs(fd - gv) / (rnd + 1e-6 * scale) if rnd + scale > 0 else 0.0
+            low_ok += r <= 1.0
+            low_worst = max(low_worst, r)
             continue
         checked += 1
         rel = abs(fd - gv) / scale
         strict += rel <= 1e-6
         worst = max(worst, rel)
     pc.set_fields(fields)
-    ok = checked == ncheck and strict == checked
+    ok = checked == ncheck and strict == checked and low_ok == unresolved
     log(f"[bench] central differences on {checked} cells ({min(nrays)}-{max(nrays)} rays "
         f"each; {unresolved} drawn cells below the rounding resolution redrawn): "
-        f"{strict} within 1e-6 relative (max {worst:.2e})")
+        f"{strict} within 1e-6 relative (max {worst:.2e}); redrawn cells within their "
+        f"rounding bound: {low_ok}/{unresolved} (max err/bound {low_worst:.2e})")
     return {"cells": checked, "within_1e-6": int(strict), "max_rel_err": worst,
             "all_within_1e-6": bool(ok), "redrawn_below_resolution": unresolved,
+            "redrawn_within_rounding_bound": int(low_ok),
+            "redrawn_max_err_over_bound": low_worst,
             "step": "1e-3 relative" if mode == "sum" else "1e-3 relative (RGB), 1e-4 (alpha)",
             "resolution": "worst-case rounding of the difference <= 1e-7 max(|fd|, |g|)",
             "rays_per_cell": [min(nrays), max(nrays)]}
