@@ -135,3 +135,36 @@ def test_lattice_degenerate_rays(tuning):
     s = np.array([r[0] for r in LATTICE_RAYS], dtype=np.float64)
     e = np.array([r[1] for r in LATTICE_RAYS], dtype=np.float64)
     _check_settings(pc, s, e, ["integrate", "max", "min"], "lattice")
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23, 24, 25, 26])
+def test_random_meshes_and_rays(tuning, seed):
+    """Random clustered meshes (periodic and not) and random chords: helping
+    on (any pair given) and off give the same bits, and both equal the oracle."""
+    rng = np.random.default_rng(seed)
+    periodic = bool(seed % 2)
+    n = int(rng.integers(5_000, 40_000))
+    pos, rho = synthetic.clustered_mesh(n, seed=seed, n_clumps=int(rng.integers(3, 20)))
+    bb = [0, 100, 0, 100, 0, 100]
+    pc = vt.ProjectionCloud(pos, rho, boundbox=bb, periodic=periodic)
+    nr = int(rng.integers(500, 3000))
+    if periodic:  # axis-aligned rays across the box (the reference's periodic use)
+        xy = rng.uniform(0.0, 100.0, (nr, 2))
+        ax = rng.integers(0, 3)
+        s = np.insert(xy, ax, 0.0, axis=1)
+        e = np.insert(xy, ax, 100.0, axis=1)
+    else:
+        s = rng.uniform(5.0, 95.0, (nr, 3))
+        e = rng.uniform(5.0, 95.0, (nr, 3))
+    out = {}
+    for he in (0, 128):
+        vt.tuning.set_tuning(help_events=he)
+        _say(f"random mesh seed {seed}: {n} points, {nr} rays, periodic {periodic}, "
+             f"help_events {he}")
+        out[he] = (pc.projection(s, e, reduction="integrate"),
+                   pc.traced_projection(s, e, flatten=True))
+    np.testing.assert_array_equal(out[128][0], out[0][0])
+    for a, b in zip(out[128][1], out[0][1]):
+        np.testing.assert_array_equal(a, b)
+    oc = port.OracleCloud(pos, rho, bb, periodic=periodic)
+    np.testing.assert_array_equal(out[128][0], oc.project(s, e, "sum")[:, 0])
