@@ -78,42 +78,42 @@ constexpr int kRecChunk = 32;
 // values: the backward pass writes both to the same place.
 template <class InnerIO>
 struct RecordIO {
-  struct Sink {
-    typename InnerIO::Sink in;
-    RecPool pool;
+  // The inner sink with recording added; its segment count `n` is the
+  // place of the next segment along the ray.
+  struct Sink : InnerIO::Sink {
+    using Base = typename InnerIO::Sink;
+    RecPool pool;  // set once per lane before the first ray (setup), the same in every lane
     uint32_t ray;
-    uint64_t n;
-    unsigned long long base = 0;
-    int fill = kRecChunk;
+    // Next record of this lane's current chunk; a multiple of kRecChunk
+    // means the chunk is used up (chunks start at multiples of kRecChunk).
+    unsigned long long next = 0;
     __device__ __forceinline__ void seg(uint32_t cell, double a, double b) {
-      in.seg(cell, a, b);
-      if (fill == kRecChunk) {
-        base = atomicAdd(pool.head, (unsigned long long)kRecChunk);
-        fill = 0;
-      }
-      const unsigned long long i = base + (unsigned long long)fill;
-      if (i < pool.cap) {
-        SegRec *r = pool.rec + i;
+      const uint32_t k = (uint32_t)this->n;  // the segment's place along the ray
+      Base::seg(cell, a, b);
+      if ((next & (unsigned long long)(kRecChunk - 1)) == 0ull)
+        next = atomicAdd(pool.head, (unsigned long long)kRecChunk);
+      if (next < pool.cap) {
+        SegRec *r = pool.rec + next;
         // Records are read only by a later kernel.  Measured at 2M clustered
         // cells, 512^2: HBM reads of the recording forward 13.7 -> 8.0 GB.
-        gst_nt(reinterpret_cast<uint4 *>(r), make_uint4(cell, ray, (uint32_t)n, 0u));
+        gst_nt(reinterpret_cast<uint4 *>(r), make_uint4(cell, ray, k, 0u));
         gst_nt(&r->ds, b - a);
       } else {
         atomicOr(pool.overflow, 1);
       }
-      ++fill;
-      ++n;
+      ++next;
     }
   };
   InnerIO inner;
   RecPool pool;
+  __device__ void setup(Sink &sk) const { sk.pool = pool; }
   __device__ void begin(int64_t ray, Sink &sk) const {
-    inner.begin(ray, sk.in);
-    sk.pool = pool;
+    inner.begin(ray, static_cast<typename Sink::Base &>(sk));
     sk.ray = (uint32_t)ray;
-    sk.n = 0;
   }
-  __device__ int end(int64_t ray, Sink &sk) const { return inner.end(ray, sk.in); }
+  __device__ int end(int64_t ray, Sink &sk) const {
+    return inner.end(ray, static_cast<typename Sink::Base &>(sk));
+  }
 };
 
 // Recorded segments -> the ray-major (slot, ds) lists at offsets[ray] + k.

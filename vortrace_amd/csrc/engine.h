@@ -506,6 +506,17 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int n) {
 // Lane roles.
 enum : int { kRoleIdle = 0, kRoleOwner = 1, kRoleHelper = 2 };
 
+// io.setup(sink), when the IO has one: state every lane's sink holds for the
+// whole launch, set once before the first ray (the same values in every
+// lane, so the compiler keeps them in scalar registers).
+template <class IO, class Sink>
+__device__ __forceinline__ auto io_setup(const IO &io, Sink &sk, int)
+    -> decltype(io.setup(sk), void()) {
+  io.setup(sk);
+}
+template <class IO, class Sink>
+__device__ __forceinline__ void io_setup(const IO &, Sink &, long) {}
+
 // Runs rays through the walk, handing segments to the sinks IO provides:
 // IO::Sink, io.begin(ray, sink), io.end(ray, sink) -> error code (kErrNone or
 // a failure of the ray's output).  The rays are [0, nrays), or ray_list[0,
@@ -527,6 +538,7 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
   Walker<kPer> W;
   Search<kPer> Q;
   LaneSink<typename IO::Sink> sk;
+  io_setup(io, sk.in, 0);
   sk.ev = S.help_ev ? S.help_ev + wbase : nullptr;
   sk.stride = S.lanes;
   sk.cap = S.help_cap;
