@@ -309,10 +309,10 @@ def fwd_adjoint(pc, fields, d_start, d_end, args, C, R, mode="sum"):
 
     split = {"forward_ms": [], "backward_ms": []}
 
-    def step():
+    def step(rs=d_start, re=d_end):
         f = f_t.clone().requires_grad_(True)
         t0 = time.perf_counter()
-        p = vt.autograd.project(pc, f, d_start, d_end, mode)
+        p = vt.autograd.project(pc, f, rs, re, mode)
         loss = ((p - target) ** 2).sum()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -323,11 +323,19 @@ def fwd_adjoint(pc, fields, d_start, d_end, args, C, R, mode="sum"):
         return f.grad, pc._forward_stats, pc._adjoint_stats
 
     first = None  # the first pass over these rays (record pool from a sample)
+    if args.warmup > 0:
+        # The process's first autograd step, on copies of the rays (a ray set
+        # of its own): one-time costs of torch's autograd, the loss kernels and
+        # the first record pool, kept out of the first pass over the step's rays.
+        step(d_start.clone(), d_end.clone())
+        first = {"process_first_step_ms": split["forward_ms"][0] + split["backward_ms"][0]}
+        split["forward_ms"].clear()
+        split["backward_ms"].clear()
     for i in range(args.warmup):
         step()
-        if first is None:
-            first = {"ms": split["forward_ms"][0] + split["backward_ms"][0],
-                     "adjoint_path": pc._adjoint_path}
+        if i == 0:
+            first.update({"ms": split["forward_ms"][0] + split["backward_ms"][0],
+                          "adjoint_path": pc._adjoint_path})
         log(f"[bench] fwd+adjoint warmup {i + 1}/{args.warmup} done")
     split["forward_ms"].clear()
     split["backward_ms"].clear()

@@ -248,7 +248,7 @@ def test_autograd_first_pass_sampled_pool(monkeypatch):
     walk the rays.  The gradient has the same bits either way."""
     _say("autograd first pass, sampled pool")
     pc, f_np, ds_, de_, _ = _recorded_case("sum")
-    monkeypatch.setattr(vt.autograd, "_SAMPLE_RAYS", 16)  # every 6th of 1600 rays
+    monkeypatch.setattr(vt.autograd, "_SAMPLE_RAYS", 256)  # every 6th of 1600 rays
     f0 = torch.from_numpy(f_np).to(ds_.device)
     w = torch.from_numpy(np.random.default_rng(36).standard_normal((ds_.shape[0], 2))).to(
         ds_.device)

@@ -17,7 +17,7 @@ of walking the rays again (``vt_project_recorded`` / ``vt_adjoint_recorded``).
 The pool is sized from the segment count of the previous forward pass over
 the same rays (the same ray tensors or arrays, unchanged; a few ray sets are
 remembered per cloud), or on the first pass over a ray set from a sample
-(every k-th ray, about 65,536 of them, projected on its own); when the pool
+(every k-th ray, about 4,096 of them, projected on its own); when the pool
 overflows, the backward pass walks the rays for their segments
 (``vt_adjoint``).  Both give the same bits.
 
@@ -55,17 +55,20 @@ def _pool_capacity(estimate: int, lanes: int) -> int:
     return int(estimate * _POOL_GROWTH) + 2 * lanes * _REC_CHUNK
 
 
-# First pass over a ray set: segments estimated from every k-th ray (at least
+# First pass over a ray set: segments estimated from every k-th ray (about
 # _SAMPLE_RAYS of them), scaled up with headroom for the sampling error.
+# Measured on 2M and 1M clustered meshes (512^2 and 256^2 z-rays): 2,048 to
+# 4,096 sampled rays put the total within 0.5 % of the true one, and their
+# projection takes 2-3 ms (65,536 of them took 21 ms).
 _SAMPLE_RAYS = 4096
 _SAMPLE_HEADROOM = 1.25
 
 
 def _sampled_estimate(cloud, mode, pos_start, pos_end, n: int) -> int:
     """Segments of all n rays, estimated by projecting every k-th ray."""
-    k = max(1, n // (16 * _SAMPLE_RAYS))
-    if k == 1:
-        return -1  # few rays: no estimate, the first pass walks twice
+    k = n // _SAMPLE_RAYS
+    if k < 2:
+        return -1  # too few rays to sample: the first pass walks twice
     s = torch.as_tensor(pos_start).reshape(-1, 3)[::k]
     e = torch.as_tensor(pos_end).reshape(-1, 3)[::k]
     p = Projection(s, e)
