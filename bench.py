@@ -357,7 +357,9 @@ def fwd_adjoint(pc, fields, d_start, d_end, args, C, R, mode="sum"):
     rep = {"ms_per_step": ms, "crossings_per_s": C / (ms * 1e-3), "rays_per_s": R / (ms * 1e-3),
            "forward_kernel_ms": float(np.mean(fms)),
            "adjoint_kernel_ms": float(np.mean(kms)), "forward_ms": fwd_ms,
-           "backward_ms": bwd_ms, "cells_with_gradient": nz, "first_step": first,
+           "backward_ms": bwd_ms,
+           "step_ms_median": float(np.median(np.add(split["forward_ms"], split["backward_ms"]))),
+           "cells_with_gradient": nz, "first_step": first,
            "step": "L = sum (proj(fields) - target)^2 and dL/dfields by torch.autograd "
                    "(forward kernel recording its segments + adjoint kernels over the "
                    "records), FP64"}
@@ -598,7 +600,9 @@ def main():
         f"{st.slow_queries} K_EXPAND, walk depth {st.max_walk_depth}, "
         f"kNN stack {st.max_knn_depth}")
     fwd = {"ms_per_step": ms_per_step, "crossings_per_s": C / (ms_per_step * 1e-3),
-           "rays_per_s": R / (ms_per_step * 1e-3), "kernel_ms": kernel_ms}
+           "rays_per_s": R / (ms_per_step * 1e-3), "kernel_ms": kernel_ms,
+           "kernel_ms_median": float(np.median(kms)), "kernel_ms_min": float(min(kms)),
+           "kernel_ms_max": float(max(kms))}
 
     fa = None
     if kind == "fit" or (mode == "sum" and not args.no_adjoint):
