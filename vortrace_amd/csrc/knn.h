@@ -18,7 +18,7 @@
 //    levels, which becomes the stack once the first leaf is scanned.  Only if
 //    an entry that fell out of the ring could still be visited is the path
 //    walked again.
-//  * Every entry records a lower bound (a float rounded down) of the box
+//  * Every entry records a lower bound (a float at or below it) of the box
 //    distances of the entries below it.  Once that bound exceeds the worst
 //    distance, nanoflann would prune everything still pending, so the search
 //    ends there.
@@ -88,7 +88,9 @@ struct KStack {
     base = 0;
     lo = 0;
   }
-  __device__ __forceinline__ void push(uint32_t tag, double v, float pb) {
+  // The LDS slot of a new top entry (moving the window's bottom entry to
+  // device memory first when the window is full); the caller writes it.
+  __device__ __forceinline__ int push_slot() {
     if (sp - lo >= kLdsDepth) {
       if (lo < gcap)
         gst(gs + (size_t)lo * lanes, lds_entry(slot(lo)));
@@ -96,9 +98,13 @@ struct KStack {
         overflow = true;
       ++lo;
     }
-    set_lds_entry(slot(sp), pack_entry(tag, v, pb));
+    const int sl = slot(sp);
     ++sp;
     maxsp = sp > maxsp ? sp : maxsp;
+    return sl;
+  }
+  __device__ __forceinline__ void push(uint32_t tag, double v, float pb) {
+    set_lds_entry(push_slot(), pack_entry(tag, v, pb));
   }
   // Entry i (i < sp).  An entry lost to overflow reads as (kSkip, 0, 0):
   // skipped, and its bound of 0 stops nothing.
