@@ -123,14 +123,12 @@ struct Search {
     const bool low_first = choose(nd, qd(f), cut);
     const double m2 = mind + cut - ddd(f);
     const uint32_t tag = far_tag(low_first ? nd.c2 : nd.c1, f);
-    if (m2 <= rs.worst()) {
-      // The far child can still be visited: it goes on the stack, or on the
-      // first descent (no point found yet, worst = the search's bound) into
-      // the LDS ring, the depth-th at slot depth mod kLdsDepth.  One entry
-      // write for both.
-      int sl;
-      if (depth >= 0) {
-        sl = st.ring_slot(depth);
+    if (depth >= 0) {
+      // First descent (no point found yet, worst = the search's bound): every
+      // far child that can still be visited goes into the LDS ring, the
+      // depth-th at slot depth mod kLdsDepth; one beyond the bound never can.
+      if (m2 <= rs.worst()) {
+        const int sl = st.ring_slot(depth);
         if (depth >= kLdsDepth) {
           uint32_t ot;
           double old;
@@ -138,11 +136,12 @@ struct Search {
           unpack_entry(lds_entry(sl), ot, old, op);
           lost_min = old < lost_min ? old : lost_min;
         }
+        set_lds_entry(sl, pack_entry(tag, m2, ptop));
+        ptop = fminf(ptop, bound_below(m2));
         ++depth;
-      } else {
-        sl = st.push_slot();
       }
-      set_lds_entry(sl, pack_entry(tag, m2, ptop));
+    } else if (m2 <= rs.worst()) {
+      st.push(tag, m2, ptop);
       ptop = fminf(ptop, bound_below(m2));
     }
     return low_first ? nd.c1 : nd.c2;
