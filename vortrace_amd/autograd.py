@@ -115,7 +115,10 @@ class _RayState:
         self.counts = counts
 
 
-_MAX_RAY_STATES = 8
+# Ray sets remembered per cloud: enough for a fitting loop over 16 views, each
+# with its own ray tensors.  A state holds an int64 count per ray only when
+# ordering pays (at most 8 rays per resident lane, 8 MB), so at most 128 MB.
+_MAX_RAY_STATES = 16
 
 
 def _ray_states(pc) -> collections.OrderedDict:
