@@ -201,6 +201,15 @@ def test_recorded_adjoint_matches_segment_walk(case):
     counts = torch.empty(n, dtype=torch.int64, device=dev)
     pool = torch.empty((total + 2 * 131072 * 32, vt.core.SEGREC_BYTES), dtype=torch.uint8,
                        device=dev)
+    # The pool is not cleared by the call (each lane marks the unfilled rest
+    # of its last chunk): fill it with stale records that pass the reorder's
+    # checks (ray j mod n, place 0, cell 0, ds 1e300) so that one left
+    # unmarked would change the gradient.
+    stale = pool.view(torch.int32)
+    stale[:, 0] = 0
+    stale[:, 1] = (torch.arange(pool.shape[0], device=dev) % n).to(torch.int32)
+    stale[:, 2] = 0
+    pool.view(torch.float64)[:, 2] = 1e300
     img_r, used = p.project_recorded_tensor(pc._cloud, mode, counts, pool)
     deferred = vt._lib.last_call_stats().deferred_rays
     _say(f"  {n} rays, {total} segments, {used} records, {deferred} deferred rays")

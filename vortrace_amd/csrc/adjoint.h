@@ -54,8 +54,8 @@ struct SegIO {
 };
 
 // One segment recorded by a forward pass: the cell's slot, the ray, its
-// position k along the ray, and ds.  ray == kNone marks a record never
-// written (the pool is filled with 0xFF first).
+// position k along the ray, and ds.  ray == kNone marks a record of a
+// lane's last chunk that the lane did not fill.
 struct alignas(16) SegRec {
   uint32_t slot, ray, k, pad;
   double ds, pad2;
@@ -63,7 +63,10 @@ struct alignas(16) SegRec {
 static_assert(sizeof(SegRec) == 32, "segment record must be 32 bytes");
 
 // Pool the records go to: lanes take chunks of kRecChunk records from `head`
-// and fill them in order; `overflow` is set when the pool runs out.
+// and fill them in order; `overflow` is set when the pool runs out.  A lane
+// marks the records of its last chunk that it did not fill (ray = kNone)
+// after its last ray, so every record below the final head is either written
+// or marked and the pool needs no clearing beforehand.
 struct RecPool {
   SegRec *rec;
   unsigned long long cap;
@@ -107,6 +110,14 @@ struct RecordIO {
   InnerIO inner;
   RecPool pool;
   __device__ void setup(Sink &sk) const { sk.pool = pool; }
+  // After the lane's last ray: the rest of its current chunk is unwritten.
+  __device__ void finish(Sink &sk) const {
+    unsigned long long j = sk.next;
+    while ((j & (unsigned long long)(kRecChunk - 1)) != 0ull && j < pool.cap) {
+      gst(&pool.rec[j].ray, kNone);
+      ++j;
+    }
+  }
   __device__ void begin(int64_t ray, Sink &sk) const {
     inner.begin(ray, static_cast<typename Sink::Base &>(sk));
     sk.ray = (uint32_t)ray;

@@ -518,6 +518,16 @@ __device__ __forceinline__ auto io_setup(const IO &io, Sink &sk, int)
 template <class IO, class Sink>
 __device__ __forceinline__ void io_setup(const IO &, Sink &, long) {}
 
+// io.finish(sink), when the IO has one: run once by every lane after its
+// last ray of the launch.
+template <class IO, class Sink>
+__device__ __forceinline__ auto io_finish(const IO &io, Sink &sk, int)
+    -> decltype(io.finish(sk), void()) {
+  io.finish(sk);
+}
+template <class IO, class Sink>
+__device__ __forceinline__ void io_finish(const IO &, Sink &, long) {}
+
 // Runs rays through the walk, handing segments to the sinks IO provides:
 // IO::Sink, io.begin(ray, sink), io.end(ray, sink) -> error code (kErrNone or
 // a failure of the ray's output).  The rays are [0, nrays), or ray_list[0,
@@ -840,6 +850,7 @@ __device__ __forceinline__ void run_engine(const DevCloud &C, const Scratch &S,
       if (ph == kPhSearch && Q.mode <= kTmEnter) Q.node_step(C, st);
     }
   }
+  io_finish(io, sk.in, 0);
 }
 
 }  // namespace vt

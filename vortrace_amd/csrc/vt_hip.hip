@@ -1717,7 +1717,6 @@ int vt_project_recorded(const vt_cloud *cc, const double *d_start, const double 
     int *overflow = B.get<int>(1);
     HIP_OK(hipMemsetAsync(head, 0, sizeof *head, st));
     HIP_OK(hipMemsetAsync(overflow, 0, sizeof *overflow, st));
-    if (capacity) HIP_OK(hipMemsetAsync(d_records, 0xFF, capacity * sizeof(SegRec), st));
     const RecPool pool{static_cast<SegRec *>(d_records), (unsigned long long)capacity, head,
                        overflow};
     if (mode == VT_SUM) {
