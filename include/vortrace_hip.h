@@ -229,9 +229,9 @@ int vt_adjoint(const vt_cloud *cloud, const double *d_start, const double *d_end
  * the rays.  vt_project_recorded is vt_project_counted (modes VT_SUM and
  * VT_VOLUME) that also appends one VT_SEGREC_BYTES record per segment (its
  * cell, its ray and its place along the ray, ds) to d_records, a device pool
- * of `capacity` records, and sets *n_records to the records used, or to
- * (size_t)-1 when the pool was too small (the image and counts are still
- * complete).  vt_adjoint_recorded is vt_adjoint over those records: it
+ * of `capacity` records (whatever it held before: it need not be cleared),
+ * and sets *n_records to the records used, or to (size_t)-1 when the pool
+ * was too small (the image and counts are still complete).  vt_adjoint_recorded is vt_adjoint over those records: it
  * scatters them to ray-major order by d_counts and runs the same
  * contributions and per-cell sums, so the two adjoints agree bit for bit.
  * The cloud's fields must be those of the forward pass for VT_VOLUME.
